@@ -1,0 +1,285 @@
+/*
+ * loquat_amd.h -- C ABI of the MI355X-native LOQUAT SPH time step.
+ *
+ * Plain C: pointers, sizes and status codes; no HIP or torch types.  Every
+ * entry point returns an lq_status and never exits the process (the reference
+ * exits through checkCudaErrors, include/helper_cuda.h:985-1016).
+ *
+ * Module boundaries follow the reference:
+ *   loquat_grid          -> lq_grid_*                (src/loquat_grid.h:28-66)
+ *   particle_interaction -> lq_particle_interaction  (src/particle_interaction.h:29-41)
+ *   adami_boundary       -> lq_adami_boundary        (src/adami_boundary.h:29-36)
+ *   predictor_corrector  -> lq_predictor/lq_corrector(src/predictor_corrector.h:34-67)
+ *   field_smoothing      -> lq_field_smoothing       (src/field_smoothing.h:29-37)
+ *   LoquatSph time loop  -> lq_engine_*              (src/loquat.cu:216-475)
+ *   LoquatIo input side  -> lq_load_project          (src/loquat_io.cu:69-237)
+ *
+ * Device state is structure-of-arrays, sorted by cell, with FP64 positions
+ * beside the reference's FP32 float4/tensor2d arrays.  Module functions
+ * operate on caller-owned device arrays; all launches go to the given stream
+ * and are asynchronous, as in the reference (the caller synchronises).
+ */
+#ifndef LOQUAT_AMD_H
+#define LOQUAT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum lq_status {
+    LQ_OK = 0,
+    LQ_ERR_INVALID_ARG = 1,   /* null pointer, negative size, np != npb + npm, ... */
+    LQ_ERR_HIP = 2,           /* a HIP runtime call failed */
+    LQ_ERR_OUT_OF_MEMORY = 3, /* device allocation failed */
+    LQ_ERR_IO = 4,            /* project file missing or unreadable */
+    LQ_ERR_PARSE = 5,         /* malformed XML / .dat, wrong value type */
+    LQ_ERR_MISSING_KEY = 6,   /* mandatory XML key absent (src/loquat_io.cu:56-66) */
+    LQ_ERR_UNSUPPORTED = 7,   /* option the path does not implement */
+    LQ_ERR_CAPACITY = 8,      /* more particles / cells / runs than the handle holds */
+    LQ_ERR_DOMAIN = 9,        /* a wall particle lies outside the cell grid */
+    LQ_ERR_STATE = 10         /* call made in the wrong order (e.g. permute before build) */
+} lq_status;
+
+/* hipStream_t passed opaquely; NULL is the null stream. */
+typedef void* lq_stream;
+
+/* Run-time constants.  Field for field, and byte for byte (208 B), the
+ * reference's Parameters (include/types.h:89-155), so the same XML fills it
+ * and a reference Parameters* can be passed in unchanged. */
+typedef struct lq_params {
+    int32_t np, npb, npm, np_delete;
+    float domain_size_x, domain_size_y, domain_size_z;
+    float domain_min_x, domain_min_y, domain_min_z;
+    int32_t cell_num_x, cell_num_y, cell_num_z;
+    int32_t ncell, ncell_one_layer, ncell_delete;
+    float h, i_h, eta, dr, hdr;
+    float m, cs, cfl, rho0;
+    int32_t constitutive_model;   /* 1 Drucker-Prager, 2 Wu hypoplastic, 3 von Wolffersdorff */
+    float model_par_1, model_par_2, model_par_3, model_par_4, model_par_5;
+    int32_t density_shepard_filter, stress_shepard_filter;
+    int32_t dynamic_time_step, integration_method, viscosity_type;
+    float artificial_viscosity_alpha;
+    int32_t artificial_pressure;
+    float artificial_pressure_coefficient;
+    float acc_x, acc_y, acc_z;
+    float time_max, time, dt, output_frequency;
+    int32_t output_number;
+    float kernel_normalization_par, kernel_gradient_normalization_par;
+    float kernel_zero, kernel_dr;
+    int32_t output_format;
+} lq_params;
+
+/* Constants of constitutive model 3 (von Wolffersdorff hypoplasticity).  The
+ * reference has no such model; its five model_par slots cannot hold eight
+ * constants.  Angles in degrees, hs in Pa. */
+typedef struct lq_vw_params {
+    float phi_c, hs, n, ed0, ec0, ei0, alpha, beta;
+} lq_vw_params;
+
+/* Particle state on the device, sorted by cell.  np entries per array.
+ *   xd, yd, zd : FP64 positions (the precise copy, integrated in FP64)
+ *   pospres    : float4 (float)x, (float)y, (float)z, pressure -- always the
+ *                FP32 rounding of xd/yd/zd; the grid and neighbour tests read it
+ *   velrhop    : float4 vx, vy, vz, density
+ *   idep       : float4 object id, equivalent plastic strain, void ratio, unused
+ *   str, stn   : tensor2d (6 floats: xx yy zz xy xz yz) stress, total strain
+ *   orig       : int32 index of the particle in the input file */
+typedef struct lq_state {
+    double* xd;
+    double* yd;
+    double* zd;
+    float* pospres;
+    float* velrhop;
+    float* idep;
+    float* str;
+    float* stn;
+    int32_t* orig;
+} lq_state;
+
+/* Start-of-step copy read by the corrector (the reference's *_ini arrays,
+ * src/loquat.h:66-71, reduced to the fields Corrector_cuk reads,
+ * src/predictor_corrector.cu:269-290).  epvr is float2 (ep, void ratio). */
+typedef struct lq_state_ini {
+    double* xd;
+    double* yd;
+    double* zd;
+    float* velrhop;
+    float* epvr;
+    float* str;
+    float* stn;
+} lq_state_ini;
+
+/* Rates produced by the interaction (src/loquat.h:73-76):
+ *   accad  float4 ax, ay, az, drho/dt;  spin float4 (W_yx, W_zx, W_zy, 0);
+ *   acestn tensor2d strain rate. */
+typedef struct lq_rates {
+    float* accad;
+    float* spin;
+    float* acestn;
+} lq_rates;
+
+/* ------------------------------------------------------------------ params */
+
+/* InitializeParameters (src/loquat.cu:111-139): h, i_h, eta, m, cell counts,
+ * ncell, Wendland constants, and dt = cfl*h/cs (src/loquat.cu:331-333). */
+lq_status lq_params_derive(lq_params* par);
+
+/* FP32 squared-distance threshold T: r2 < T  <=>  sqrtf(r2) * i_h < 2
+ * (neighbour predicate q < 2 of src/adami_boundary.cu:41-43). */
+lq_status lq_neighbour_r2_cut(const lq_params* par, float* r2_cut);
+
+/* ---------------------------------------------------------------- project */
+
+/* Host particle arrays in the reference's AoS layout (src/loquat_io.cu:207-213):
+ * pospres/velrhop/idep float4[np], str/stn tensor2d[np]. */
+typedef struct lq_host_particles {
+    float* pospres;
+    float* velrhop;
+    float* idep;
+    float* str;
+    float* stn;
+} lq_host_particles;
+
+/* LoquatIo::LoadParameters (src/loquat_io.cu:69-201): reads <prefix>.xml.
+ * Missing keys, wrong types and np != npb + npm return errors. */
+lq_status lq_load_parameters(const char* prefix, lq_params* par, lq_vw_params* vw);
+
+/* LoquatIo::LoadParticles (src/loquat_io.cu:204-237): reads par->np rows of 22
+ * columns from <prefix>.dat into caller-allocated host arrays. */
+lq_status lq_load_particles(const char* prefix, const lq_params* par, lq_host_particles* out);
+
+/* Last error message of the calling thread (empty if none). */
+const char* lq_last_error(void);
+
+/* ------------------------------------------------------------ loquat_grid */
+
+typedef struct lq_grid lq_grid;
+
+/* LoquatGrid() + GridConfig + ResizeArrayLength (src/loquat_grid.h:47-58). */
+lq_status lq_grid_create(int64_t np_cap, int64_t ncell_cap, lq_grid** out);
+lq_status lq_grid_destroy(lq_grid* grid);
+
+/* NeighborSearch (src/loquat_grid.h:60): cell hash of every particle from
+ * pospres, stable radix sort of (cell, index), cell_beg/cell_end table
+ * (cell_beg = -1 and cell_end = -1 for empty cells). */
+lq_status lq_grid_build(lq_grid* grid, const lq_params* par, const float* pospres, lq_stream stream);
+
+/* SortArraysPrimary / SortArraysAll (src/loquat.cu:190-213) through
+ * SortArrayFloat4 / SortArrayTensor2d (src/loquat_grid.h:62-64):
+ * dst[k] = src[part_idx[k]] for every state array; ini_src/ini_dst may be
+ * NULL (Primary) or both set (All).  src and dst must not alias. */
+lq_status lq_grid_permute(lq_grid* grid, int32_t np,
+                          const lq_state* src, const lq_state* dst,
+                          const lq_state_ini* ini_src, const lq_state_ini* ini_dst,
+                          lq_stream stream);
+
+/* GetPartCel / GetPartIdx / GetCellBeg / GetCellEnd (src/loquat_grid.h:51-54):
+ * device pointers valid until the next build or destroy. */
+lq_status lq_grid_arrays(const lq_grid* grid, const int32_t** part_cel, const int32_t** part_idx,
+                         const int32_t** cell_beg, const int32_t** cell_end);
+
+/* ------------------------------------------------------- module kernels */
+
+/* AdamiBoundary (src/adami_boundary.cu:126-143): stress of wall particles
+ * [0, npb) extrapolated from material neighbours. */
+lq_status lq_adami_boundary(const lq_params* par, const lq_state* st,
+                            const int32_t* cell_beg, const int32_t* cell_end, lq_stream stream);
+
+/* ParticleInteraction (src/particle_interaction.h:29-41): accad, spin and
+ * acestn of material particles [npb, np).  nbr_count (nullable, int32[np])
+ * receives each material particle's neighbour count (j != i, q < 2). */
+lq_status lq_particle_interaction(const lq_params* par, const lq_state* st, const lq_rates* rates,
+                                  const int32_t* cell_beg, const int32_t* cell_end,
+                                  int32_t* nbr_count, lq_stream stream);
+
+/* Predictor (src/predictor_corrector.cu:221-253): copies the material state
+ * into ini, then advances it half a step with the constitutive update.
+ * vw is read only when constitutive_model == 3 (may be NULL otherwise). */
+lq_status lq_predictor(const lq_params* par, const lq_vw_params* vw, const lq_state* st,
+                       const lq_state_ini* ini, const lq_rates* rates, lq_stream stream);
+
+/* Corrector (src/predictor_corrector.cu:310-336): midpoint update from ini. */
+lq_status lq_corrector(const lq_params* par, const lq_vw_params* vw, const lq_state* st,
+                       const lq_state_ini* ini, const lq_rates* rates, lq_stream stream);
+
+/* StressRegularization (src/field_smoothing.cu:124-145): Shepard filter of
+ * material stress; str_cache is tensor2d[np] scratch; result lands in st->str. */
+lq_status lq_field_smoothing(const lq_params* par, const lq_state* st, float* str_cache,
+                             const int32_t* cell_beg, const int32_t* cell_end, lq_stream stream);
+
+/* ----------------------------------------------------------------- engine
+ * The time loop of LoquatSph::RunSimulation (src/loquat.cu:322-409) for one
+ * or many independent runs packed in one segmented layout: every stage is one
+ * launch over all resident runs, each step is replayed from a hipGraph, and
+ * runs join and leave between steps (continuous batching).  A single run is a
+ * batch of one. */
+
+typedef struct lq_engine lq_engine;
+
+typedef struct lq_run_input {
+    lq_params par;               /* as loaded; lq_params_derive is applied by add_run */
+    lq_vw_params vw;             /* model 3 only */
+    lq_host_particles host;      /* reference-layout host arrays, np rows */
+    const double* posd;          /* optional FP64 positions, double[3*np] xyz; NULL = widen pospres */
+    int64_t nsteps;              /* steps to run; <= 0: until time >= time_max (src/loquat.cu:322) */
+} lq_run_input;
+
+/* Capacities bound the packed layout: particles, cells (sum over runs), runs. */
+lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t max_runs, lq_engine** out);
+lq_status lq_engine_destroy(lq_engine* eng);
+
+/* Uploads a run, derives its parameters, performs the initial neighbour
+ * search and Primary sort (src/loquat.cu:297-298).  run_id is stable for the
+ * run's lifetime. */
+lq_status lq_engine_add_run(lq_engine* eng, const lq_run_input* in, int32_t* run_id);
+
+/* Removes a run and compacts the layout (the other runs' results are not
+ * affected: each run's arithmetic depends only on its own particles). */
+lq_status lq_engine_remove_run(lq_engine* eng, int32_t run_id);
+
+/* Advances every unfinished run by up to nsteps steps (runs stop at their own
+ * nsteps).  Asynchronous on the engine stream. */
+lq_status lq_engine_step(lq_engine* eng, int32_t nsteps);
+lq_status lq_engine_synchronize(lq_engine* eng);
+
+/* Host-side progress of one run: steps done, FP64 time, finished flag. */
+lq_status lq_engine_run_info(const lq_engine* eng, int32_t run_id, int64_t* steps_done,
+                             double* time, int32_t* finished, int32_t* np);
+
+/* Copies one run's state back in input-file order (reference layout host
+ * arrays, np rows; posd double[3*np] optional).  Synchronous. */
+lq_status lq_engine_download(lq_engine* eng, int32_t run_id, lq_host_particles* out, double* posd);
+
+/* Sum over resident runs: material particles, all particles, cells. */
+lq_status lq_engine_totals(const lq_engine* eng, int64_t* npm, int64_t* np, int64_t* ncell, int32_t* nruns);
+
+/* Device bytes one run of np particles and ncell cells occupies in an engine
+ * (used for admission by HBM footprint). */
+int64_t lq_engine_bytes_per_run(int64_t np, int64_t ncell);
+
+/* Engine stream (hipStream_t) and options. */
+lq_stream lq_engine_stream(const lq_engine* eng);
+/* use_graphs: 1 = replay each step from a captured hipGraph (default), 0 = direct launches */
+lq_status lq_engine_set_graphs(lq_engine* eng, int32_t use_graphs);
+
+/* Mean device time per step of each stage over the last lq_engine_profile
+ * call: grid (hash+sort+table+permute), adami, interaction, integrate,
+ * shepard, in milliseconds.  Runs nsteps steps with events between stages
+ * (no graphs). */
+lq_status lq_engine_profile(lq_engine* eng, int32_t nsteps, double* ms_stage5);
+
+/* Interaction kernel variant for the engine and lq_particle_interaction:
+ * 0 = per-thread direct loop, 1 = LDS-compacted neighbour lists (default). */
+lq_status lq_set_interaction_variant(int32_t variant);
+int32_t lq_get_interaction_variant(void);
+
+/* Build/version string. */
+const char* lq_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOQUAT_AMD_H */

@@ -1,0 +1,152 @@
+// lq_common.hpp -- device-side types and helpers shared by the HIP kernels.
+//
+// Layout: every resident run owns one contiguous particle segment
+// [p_off, p_off + np) of the packed SoA arrays -- walls first, then material
+// particles, as the reference's index convention requires (walls are [0, npb),
+// src/adami_boundary.cu:73, src/predictor_corrector.cu:182-184) -- and one
+// contiguous range of cell ids [cell_base, cell_base + ncell).  A single run
+// is a batch of one.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/loquat_amd.h"
+
+namespace lq {
+
+// Symmetric second-order tensor, stored xx yy zz xy xz yz (include/types.h:36-38).
+struct alignas(8) tensor2d {
+    float xx, yy, zz, xy, xz, yz;
+};
+static_assert(sizeof(tensor2d) == 24, "tensor2d must stay 24 bytes");
+
+// Per-run constants and counters, device resident.  par is the reference's
+// Parameters (already derived); the rest describes the run's segment.
+struct RunDesc {
+    lq_params par;        // 208 B
+    int32_t p_off;        // first particle of the run in the packed arrays
+    int32_t np, npb, npm; // particles, walls, material
+    int32_t cell_base;    // first global cell id of the run
+    float r2_cut;         // neighbour threshold on FP32 squared distance
+    int32_t active;       // 1 while the run still steps
+    int32_t shepard_due;  // this step applies the stress Shepard filter
+    int32_t step;         // steps completed
+    int32_t nsteps;       // steps requested
+    double time;          // FP64 simulated time (src/loquat.cu:305,455)
+    lq_vw_params vw;      // constitutive model 3 constants
+    float vw_a, vw_fb0;   // derived model-3 constants (a, and the f_b prefactor)
+};
+
+// Descriptor access for one run passed by value (module-level calls).
+struct DescOne {
+    RunDesc d;
+    __device__ __forceinline__ int run(int) const { return 0; }
+    __device__ __forceinline__ const RunDesc& get(int) const { return d; }
+};
+
+// Descriptor access through a device table and a per-particle run id.
+struct DescTable {
+    const RunDesc* d;
+    const int32_t* run_of;
+    __device__ __forceinline__ int run(int k) const { return run_of[k]; }
+    __device__ __forceinline__ const RunDesc& get(int r) const { return d[r]; }
+};
+
+struct DevState {
+    double* xd;
+    double* yd;
+    double* zd;
+    float4* pospres;
+    float4* velrhop;
+    float4* idep;
+    tensor2d* str;
+    tensor2d* stn;
+    int32_t* orig;
+};
+
+struct DevIni {
+    double* xd;
+    double* yd;
+    double* zd;
+    float4* velrhop;
+    float2* epvr;
+    tensor2d* str;
+    tensor2d* stn;
+};
+
+struct DevRates {
+    float4* accad;
+    float4* spin;
+    tensor2d* acestn;
+};
+
+inline DevState to_dev(const lq_state& s) {
+    return DevState{s.xd, s.yd, s.zd, (float4*)s.pospres, (float4*)s.velrhop, (float4*)s.idep,
+                    (tensor2d*)s.str, (tensor2d*)s.stn, s.orig};
+}
+inline DevIni to_dev(const lq_state_ini& s) {
+    return DevIni{s.xd, s.yd, s.zd, (float4*)s.velrhop, (float2*)s.epvr, (tensor2d*)s.str, (tensor2d*)s.stn};
+}
+inline DevRates to_dev(const lq_rates& r) {
+    return DevRates{(float4*)r.accad, (float4*)r.spin, (tensor2d*)r.acestn};
+}
+
+// Cell coordinate of one axis: floorf(0.5f * x * i_h) evaluated left to right
+// in FP32 exactly as the lookups of src/adami_boundary.cu:80-82 do.  The
+// result is clamped to [-2, 2^30] (NaN -> -2) so that out-of-domain particles
+// give a well-defined, empty or partial stencil instead of an undefined
+// float->int conversion.
+__device__ __forceinline__ int cell_coord(float x, float i_h) {
+#pragma clang fp contract(off)
+    float f = floorf((0.5f * x) * i_h);
+    if (!(f >= -2.0f)) return -2;
+    if (f > 1073741824.0f) return 1073741824;
+    return (int)f;
+}
+
+// FP32 squared distance with a fixed operation order and no contraction; the
+// neighbour predicate is r2 < RunDesc::r2_cut.
+__device__ __forceinline__ float r2_fp32(float dx, float dy, float dz) {
+#pragma clang fp contract(off)
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+// Calls f(jb, je) for the contiguous particle range of each (x, y) column of
+// the 3x3x3 stencil around (cix, ciy, ciz), for cell layers [layer_lo,
+// layer_hi].  Cells of one column are consecutive ids (cell = (cx*ny + cy)*nz
+// + cz, src/adami_boundary.cu:98), so the up-to-three z cells of a column map
+// to one range of the sorted particles.  Cells outside the grid are skipped
+// as in src/adami_boundary.cu:90-96.
+template <class F>
+__device__ __forceinline__ void for_each_column(const RunDesc& d, int cix, int ciy, int ciz,
+                                                int layer_lo, int layer_hi, const int32_t* __restrict__ beg,
+                                                const int32_t* __restrict__ end, F&& f) {
+    const int nx = d.par.cell_num_x, ny = d.par.cell_num_y, nz = d.par.cell_num_z;
+    const int czlo = max(ciz - 1, 0);
+    const int czhi = min(ciz + 1, nz - 1);
+    if (czlo > czhi) return;
+    for (int layer = layer_lo; layer <= layer_hi; ++layer) {
+        const int lbase = d.cell_base + layer * d.par.ncell_one_layer;
+        for (int ox = -1; ox <= 1; ++ox) {
+            const int cx = cix + ox;
+            if (cx < 0 || cx >= nx) continue;
+            for (int oy = -1; oy <= 1; ++oy) {
+                const int cy = ciy + oy;
+                if (cy < 0 || cy >= ny) continue;
+                const int c0 = lbase + (cx * ny + cy) * nz;
+                int jb = -1, je = -1;
+                for (int cz = czlo; cz <= czhi; ++cz) {
+                    const int b = beg[c0 + cz];
+                    if (b != -1) {
+                        if (jb < 0) jb = b;
+                        je = end[c0 + cz];
+                    }
+                }
+                if (jb >= 0) f(jb, je);
+            }
+        }
+    }
+}
+
+}  // namespace lq

@@ -1,0 +1,870 @@
+// lq_engine.hip -- the C ABI: module entry points (one run, caller-owned
+// arrays) and the engine that runs the reference's time loop for one or many
+// runs packed in one segmented layout.
+//
+// Engine step (src/loquat.cu:322-409), every stage one launch over all runs:
+//   step_begin                       per-run Shepard flag (src/loquat.cu:400)
+//   [predict] grid build, permute Primary (src/loquat.cu:345-347),
+//             Adami, interaction, predictor (+ fused *_ini copy)
+//   [correct] grid build, permute All (src/loquat.cu:373-375),
+//             Adami, interaction, corrector
+//   [shepard] if any run is due: grid build, permute Primary with the stress
+//             routed to scratch, Shepard filter back into the state
+//   step_end                         per-run time += dt, step++, finish flag
+// The state ping-pongs between two buffers; each (buffer, shepard) pair has
+// its own captured hipGraph, rebuilt when runs join or leave.
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "lq_internal.hpp"
+
+namespace lq {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+static int g_variant = 1;
+int interaction_variant() { return g_variant; }
+
+// Smallest FP32 value T with sqrtf(T) * i_h >= 2 (both correctly rounded):
+// r2 < T is then exactly the predicate q = sqrtf(r2) * i_h < 2 for every
+// FP32 r2, since both operations are monotone.
+float neighbour_r2_cut(float i_h) {
+    uint32_t lo = 0, hi = 0x7f800000u;  // [0, +inf]
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        float v;
+        std::memcpy(&v, &mid, 4);
+        volatile float r = sqrtf(v);
+        volatile float q = r * i_h;
+        if (q >= 2.0f)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    float t;
+    std::memcpy(&t, &lo, 4);
+    return t;
+}
+
+static double deg2rad(double x) { return x * 3.14159265358979323846 / 180.0; }
+
+void fill_desc_static(RunDesc& d, const lq_params& par, const lq_vw_params* vw) {
+    std::memset(&d, 0, sizeof(d));
+    d.par = par;
+    d.np = par.np;
+    d.npb = par.npb;
+    d.npm = par.npm;
+    d.r2_cut = neighbour_r2_cut(par.i_h);
+    d.active = 1;
+    d.shepard_due = 1;
+    d.nsteps = 0x7fffffff;
+    d.time = (double)par.time;
+    if (vw) {
+        d.vw = *vw;
+        if (par.constitutive_model == 3 && vw->phi_c > 0.0f) {
+            const double sp = sin(deg2rad(vw->phi_c));
+            const double a = sqrt(3.0) * (3.0 - sp) / (2.0 * sqrt(2.0) * sp);
+            const double ratio = (double)(vw->ei0 - vw->ed0) / (double)(vw->ec0 - vw->ed0);
+            const double den = 3.0 + a * a - a * sqrt(3.0) * pow(ratio, (double)vw->alpha);
+            d.vw_a = (float)a;
+            d.vw_fb0 = (float)((double)vw->hs / (double)vw->n * pow((double)vw->ei0 / (double)vw->ec0, (double)vw->beta) / den);
+        }
+    }
+}
+
+}  // namespace lq
+
+using namespace lq;
+
+// ------------------------------------------------------------------ helpers
+static lq_status check_params(const lq_params* par) {
+    if (!par) {
+        set_error("null params");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (par->np < 0 || par->npb < 0 || par->npm < 0 || par->np != par->npb + par->npm) {
+        set_error("invalid particle counts: np must equal npb + npm");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (par->ncell <= 0 || par->ncell != 2 * par->ncell_one_layer + 1 ||
+        (int64_t)par->cell_num_x * par->cell_num_y * par->cell_num_z != par->ncell_one_layer) {
+        set_error("cell counts are not derived (call lq_params_derive)");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (!(par->i_h > 0.0f) || !(par->m > 0.0f)) {
+        set_error("h / m not derived (call lq_params_derive)");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (par->artificial_pressure == 1) {
+        set_error("artificial-pressure = 1 is not implemented");
+        return LQ_ERR_UNSUPPORTED;
+    }
+    if (par->constitutive_model < 1 || par->constitutive_model > 3) {
+        set_error("constitutive-model must be 1 (Drucker-Prager), 2 (Wu hypoplastic) or 3 (von Wolffersdorff)");
+        return LQ_ERR_UNSUPPORTED;
+    }
+    return LQ_OK;
+}
+
+static bool state_ok(const lq_state* s) {
+    return s && s->xd && s->yd && s->zd && s->pospres && s->velrhop && s->idep && s->str && s->stn && s->orig;
+}
+static bool ini_ok(const lq_state_ini* s) {
+    return s && s->xd && s->yd && s->zd && s->velrhop && s->epvr && s->str && s->stn;
+}
+static bool rates_ok(const lq_rates* r) { return r && r->accad && r->spin && r->acestn; }
+
+static RunDesc one_desc(const lq_params* par, const lq_vw_params* vw) {
+    RunDesc d;
+    fill_desc_static(d, *par, vw);
+    return d;
+}
+
+static lq_status last_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return LQ_ERR_HIP;
+    }
+    return LQ_OK;
+}
+
+extern "C" const char* lq_last_error(void) { return g_last_error.c_str(); }
+extern "C" const char* lq_version(void) { return "loquat_amd 1.0 (gfx950)"; }
+
+extern "C" lq_status lq_neighbour_r2_cut(const lq_params* par, float* r2_cut) {
+    if (!par || !r2_cut || !(par->i_h > 0.0f)) {
+        set_error("lq_neighbour_r2_cut: invalid argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    *r2_cut = neighbour_r2_cut(par->i_h);
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_set_interaction_variant(int32_t v) {
+    if (v != 0 && v != 1) {
+        set_error("interaction variant must be 0 or 1");
+        return LQ_ERR_INVALID_ARG;
+    }
+    g_variant = v;
+    return LQ_OK;
+}
+extern "C" int32_t lq_get_interaction_variant(void) { return g_variant; }
+
+// ------------------------------------------------------------- grid handle
+struct lq_grid {
+    GridBuffers g;
+    int n = -1;
+};
+
+extern "C" lq_status lq_grid_create(int64_t np_cap, int64_t ncell_cap, lq_grid** out) {
+    if (!out || np_cap < 0 || ncell_cap < 0 || np_cap > 0x7fffffff || ncell_cap > 0x7fffffff) {
+        set_error("lq_grid_create: invalid capacity");
+        return LQ_ERR_INVALID_ARG;
+    }
+    lq_grid* g = new lq_grid();
+    lq_status st = alloc_grid(g->g, np_cap, ncell_cap);
+    if (st != LQ_OK) {
+        delete g;
+        return st;
+    }
+    *out = g;
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_grid_destroy(lq_grid* grid) {
+    if (!grid) return LQ_OK;
+    free_grid(grid->g);
+    delete grid;
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_grid_build(lq_grid* grid, const lq_params* par, const float* pospres, lq_stream stream) {
+    if (!grid || !pospres) {
+        set_error("lq_grid_build: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    LQ_TRY(check_params(par));
+    if (par->np > grid->g.np_cap || par->ncell > grid->g.ncell_cap) {
+        set_error("lq_grid_build: np or ncell exceeds the grid capacity");
+        return LQ_ERR_CAPACITY;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const RunDesc d = one_desc(par, nullptr);
+    launch_hash_one(d, (const float4*)pospres, par->np, grid->g.keys[0], s);
+    launch_radix_sort(grid->g, par->np, rs_bits(par->ncell), s);
+    launch_cell_table(grid->g, par->np, par->ncell, s);
+    grid->n = par->np;
+    return last_launch("lq_grid_build");
+}
+
+extern "C" lq_status lq_grid_permute(lq_grid* grid, int32_t np, const lq_state* src, const lq_state* dst,
+                                     const lq_state_ini* ini_src, const lq_state_ini* ini_dst, lq_stream stream) {
+    if (!grid || !state_ok(src) || !state_ok(dst) || (ini_src && !ini_ok(ini_src)) ||
+        (ini_dst && !ini_ok(ini_dst)) || (!ini_src) != (!ini_dst)) {
+        set_error("lq_grid_permute: invalid argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (grid->n < 0 || np != grid->n) {
+        set_error("lq_grid_permute: np differs from the last lq_grid_build");
+        return LQ_ERR_STATE;
+    }
+    DevIni a, b;
+    if (ini_src) {
+        a = to_dev(*ini_src);
+        b = to_dev(*ini_dst);
+    }
+    launch_permute(grid->g.vals[grid->g.sorted_slot], np, to_dev(*src), to_dev(*dst), ini_src ? &a : nullptr,
+                   ini_dst ? &b : nullptr, nullptr, (hipStream_t)stream);
+    return last_launch("lq_grid_permute");
+}
+
+extern "C" lq_status lq_grid_arrays(const lq_grid* grid, const int32_t** part_cel, const int32_t** part_idx,
+                                    const int32_t** cell_beg, const int32_t** cell_end) {
+    if (!grid) {
+        set_error("lq_grid_arrays: null grid");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (grid->n < 0) {
+        set_error("lq_grid_arrays: no build yet");
+        return LQ_ERR_STATE;
+    }
+    if (part_cel) *part_cel = (const int32_t*)grid->g.keys[grid->g.sorted_slot];
+    if (part_idx) *part_idx = (const int32_t*)grid->g.vals[grid->g.sorted_slot];
+    if (cell_beg) *cell_beg = grid->g.cell_beg;
+    if (cell_end) *cell_end = grid->g.cell_end;
+    return LQ_OK;
+}
+
+// ------------------------------------------------------- module kernels
+extern "C" lq_status lq_adami_boundary(const lq_params* par, const lq_state* st, const int32_t* cell_beg,
+                                       const int32_t* cell_end, lq_stream stream) {
+    LQ_TRY(check_params(par));
+    if (!state_ok(st) || !cell_beg || !cell_end) {
+        set_error("lq_adami_boundary: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (par->npb == 0) return LQ_OK;  // src/adami_boundary.cu:135
+    launch_adami_one(one_desc(par, nullptr), to_dev(*st), cell_beg, cell_end, par->np, (hipStream_t)stream);
+    return last_launch("lq_adami_boundary");
+}
+
+extern "C" lq_status lq_particle_interaction(const lq_params* par, const lq_state* st, const lq_rates* rates,
+                                             const int32_t* cell_beg, const int32_t* cell_end, int32_t* nbr_count,
+                                             lq_stream stream) {
+    LQ_TRY(check_params(par));
+    if (!state_ok(st) || !rates_ok(rates) || !cell_beg || !cell_end) {
+        set_error("lq_particle_interaction: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (par->npm == 0) return LQ_OK;
+    launch_interaction_one(one_desc(par, nullptr), to_dev(*st), to_dev(*rates), cell_beg, cell_end, par->np,
+                           nbr_count, g_variant, (hipStream_t)stream);
+    return last_launch("lq_particle_interaction");
+}
+
+static lq_status integrate(bool corr, const lq_params* par, const lq_vw_params* vw, const lq_state* st,
+                           const lq_state_ini* ini, const lq_rates* rates, lq_stream stream) {
+    LQ_TRY(check_params(par));
+    if (!state_ok(st) || !ini_ok(ini) || !rates_ok(rates)) {
+        set_error("lq_predictor/lq_corrector: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (par->constitutive_model == 3 && !vw) {
+        set_error("constitutive model 3 needs lq_vw_params");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (par->npm == 0) return LQ_OK;  // src/predictor_corrector.cu:238,327
+    launch_integrate_one(corr, one_desc(par, vw), to_dev(*st), to_dev(*ini), to_dev(*rates), par->np,
+                         (hipStream_t)stream);
+    return last_launch(corr ? "lq_corrector" : "lq_predictor");
+}
+
+extern "C" lq_status lq_predictor(const lq_params* par, const lq_vw_params* vw, const lq_state* st,
+                                  const lq_state_ini* ini, const lq_rates* rates, lq_stream stream) {
+    return integrate(false, par, vw, st, ini, rates, stream);
+}
+extern "C" lq_status lq_corrector(const lq_params* par, const lq_vw_params* vw, const lq_state* st,
+                                  const lq_state_ini* ini, const lq_rates* rates, lq_stream stream) {
+    return integrate(true, par, vw, st, ini, rates, stream);
+}
+
+extern "C" lq_status lq_field_smoothing(const lq_params* par, const lq_state* st, float* str_cache,
+                                        const int32_t* cell_beg, const int32_t* cell_end, lq_stream stream) {
+    LQ_TRY(check_params(par));
+    if (!state_ok(st) || !str_cache || !cell_beg || !cell_end) {
+        set_error("lq_field_smoothing: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (par->npm == 0) return LQ_OK;  // src/field_smoothing.cu:134
+    hipStream_t s = (hipStream_t)stream;
+    DevState ds = to_dev(*st);
+    launch_shepard_one(one_desc(par, nullptr), ds, ds.str, (tensor2d*)str_cache, cell_beg, cell_end, par->np, s);
+    launch_copy_tensor((const tensor2d*)str_cache, ds.str, par->np, s);
+    return last_launch("lq_field_smoothing");
+}
+
+// ------------------------------------------------------------------ engine
+namespace {
+
+struct RunSlot {
+    int32_t id;
+    RunDesc desc;  // host mirror; dynamic fields advanced in lockstep with k_step_end
+    int64_t nsteps;
+};
+
+struct Buffers {
+    DevState S[2];
+    DevIni I[2];
+    DevRates R;
+    tensor2d* T;
+    int32_t* run_of;
+    RunDesc* desc;
+    GridBuffers grid;
+};
+
+template <class T>
+bool dmalloc(T*& p, int64_t n, std::vector<void*>& owned) {
+    void* q = nullptr;
+    if (hipMalloc(&q, (size_t)(n > 0 ? n : 1) * sizeof(T)) != hipSuccess) return false;
+    owned.push_back(q);
+    p = (T*)q;
+    return true;
+}
+
+}  // namespace
+
+struct lq_engine {
+    hipStream_t stream = nullptr;
+    int64_t np_cap = 0, ncell_cap = 0;
+    int32_t max_runs = 0;
+    Buffers b{};
+    std::vector<void*> owned;
+    std::vector<RunSlot> runs;  // layout order
+    int cur = 0;
+    int64_t np_total = 0, ncell_total = 0;
+    int32_t next_id = 0;
+    bool use_graphs = true;
+    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+};
+
+namespace {
+
+void drop_graphs(lq_engine* e) {
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            if (e->graph[i][j]) {
+                (void)hipGraphExecDestroy(e->graph[i][j]);
+                e->graph[i][j] = nullptr;
+            }
+}
+
+struct Prof {
+    hipEvent_t ev[2];
+    double ms[5];
+};
+
+struct StageTimer {
+    Prof* p;
+    hipStream_t s;
+    int stage;
+    StageTimer(Prof* p_, hipStream_t s_, int st) : p(p_), s(s_), stage(st) {
+        if (p) (void)hipEventRecord(p->ev[0], s);
+    }
+    ~StageTimer() {
+        if (!p) return;
+        (void)hipEventRecord(p->ev[1], s);
+        (void)hipEventSynchronize(p->ev[1]);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, p->ev[0], p->ev[1]);
+        p->ms[stage] += ms;
+    }
+};
+
+enum { PG_GRID = 0, PG_ADAMI = 1, PG_INTER = 2, PG_INTEG = 3, PG_SHEP = 4 };
+
+void enqueue_build(lq_engine* e, const DevState& st, hipStream_t s) {
+    Buffers& b = e->b;
+    const int n = (int)e->np_total;
+    launch_hash_table(b.desc, b.run_of, st.pospres, n, b.grid.keys[0], s);
+    launch_radix_sort(b.grid, n, rs_bits(e->ncell_total), s);
+    launch_cell_table(b.grid, n, e->ncell_total, s);
+}
+
+// One full step for every resident run, starting from buffer e->cur (the
+// caller flips cur afterwards when shepard is set).
+void enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
+    Buffers& b = e->b;
+    hipStream_t s = e->stream;
+    const int n = (int)e->np_total;
+    const int nr = (int)e->runs.size();
+    const int var = g_variant;
+    launch_step_begin(b.desc, nr, s);
+    int c = cur;
+    for (int half = 0; half < 2; ++half) {
+        const bool corr = half == 1;
+        {
+            StageTimer t(prof, s, PG_GRID);
+            enqueue_build(e, b.S[c], s);
+            if (corr)
+                launch_permute(b.grid.vals[b.grid.sorted_slot], n, b.S[c], b.S[1 - c], &b.I[0], &b.I[1], nullptr, s);
+            else
+                launch_permute(b.grid.vals[b.grid.sorted_slot], n, b.S[c], b.S[1 - c], nullptr, nullptr, nullptr, s);
+        }
+        c = 1 - c;
+        {
+            StageTimer t(prof, s, PG_ADAMI);
+            launch_adami_table(b.desc, b.run_of, b.S[c], b.grid.cell_beg, b.grid.cell_end, n, s);
+        }
+        {
+            StageTimer t(prof, s, PG_INTER);
+            launch_interaction_table(b.desc, b.run_of, b.S[c], b.R, b.grid.cell_beg, b.grid.cell_end, n, nullptr,
+                                     var, s);
+        }
+        {
+            StageTimer t(prof, s, PG_INTEG);
+            launch_integrate_table(corr, b.desc, b.run_of, b.S[c], corr ? b.I[1] : b.I[0], b.R, n, s);
+        }
+    }
+    if (shepard) {
+        {
+            StageTimer t(prof, s, PG_GRID);
+            enqueue_build(e, b.S[c], s);
+            launch_permute(b.grid.vals[b.grid.sorted_slot], n, b.S[c], b.S[1 - c], nullptr, nullptr, b.T, s);
+        }
+        c = 1 - c;
+        StageTimer t(prof, s, PG_SHEP);
+        launch_shepard_table(b.desc, b.run_of, b.S[c], b.T, b.S[c].str, b.grid.cell_beg, b.grid.cell_end, n, s);
+    }
+    launch_step_end(b.desc, nr, s);
+}
+
+// Host mirror of k_step_begin's Shepard test over all active runs.
+bool any_shepard_due(const lq_engine* e) {
+    for (const RunSlot& r : e->runs) {
+        const RunDesc& d = r.desc;
+        const int f = d.par.stress_shepard_filter;
+        if (d.active && f != 0 && d.step % f == 0 && d.step != 0) return true;
+    }
+    return false;
+}
+
+void host_step_end(lq_engine* e) {
+    for (RunSlot& r : e->runs) {
+        RunDesc& d = r.desc;
+        if (!d.active) continue;
+        d.time += (double)d.par.dt;
+        d.step += 1;
+        if (d.step >= d.nsteps) d.active = 0;
+    }
+}
+
+lq_status upload_descs(lq_engine* e) {
+    std::vector<RunDesc> h(e->runs.size());
+    for (size_t i = 0; i < e->runs.size(); ++i) h[i] = e->runs[i].desc;
+    if (!h.empty())
+        LQ_HIP_TRY(hipMemcpyAsync(e->b.desc, h.data(), h.size() * sizeof(RunDesc), hipMemcpyHostToDevice, e->stream));
+    LQ_HIP_TRY(hipStreamSynchronize(e->stream));
+    return LQ_OK;
+}
+
+// Recomputes offsets and cell bases of all runs, moving every segment of
+// buffer cur to its new place in the other buffer.
+lq_status relayout(lq_engine* e) {
+    Buffers& b = e->b;
+    hipStream_t s = e->stream;
+    LQ_HIP_TRY(hipStreamSynchronize(s));
+    const DevState& src = b.S[e->cur];
+    const DevState& dst = b.S[1 - e->cur];
+    int64_t off = 0, cbase = 0;
+    std::vector<int32_t> run_of;
+    for (size_t i = 0; i < e->runs.size(); ++i) {
+        RunDesc& d = e->runs[i].desc;
+        const int64_t o = d.p_off, np = d.np;
+        auto cp = [&](void* dp, const void* sp, size_t elem) {
+            return hipMemcpyAsync((char*)dp + off * elem, (const char*)sp + o * elem, np * elem,
+                                  hipMemcpyDeviceToDevice, s);
+        };
+        if (np > 0) {
+            LQ_HIP_TRY(cp(dst.xd, src.xd, 8));
+            LQ_HIP_TRY(cp(dst.yd, src.yd, 8));
+            LQ_HIP_TRY(cp(dst.zd, src.zd, 8));
+            LQ_HIP_TRY(cp(dst.pospres, src.pospres, 16));
+            LQ_HIP_TRY(cp(dst.velrhop, src.velrhop, 16));
+            LQ_HIP_TRY(cp(dst.idep, src.idep, 16));
+            LQ_HIP_TRY(cp(dst.str, src.str, 24));
+            LQ_HIP_TRY(cp(dst.stn, src.stn, 24));
+            LQ_HIP_TRY(cp(dst.orig, src.orig, 4));
+        }
+        d.p_off = (int32_t)off;
+        d.cell_base = (int32_t)cbase;
+        run_of.insert(run_of.end(), (size_t)np, (int32_t)i);
+        off += np;
+        cbase += d.par.ncell;
+    }
+    e->np_total = off;
+    e->ncell_total = cbase;
+    e->cur = 1 - e->cur;
+    if (!run_of.empty())
+        LQ_HIP_TRY(hipMemcpyAsync(b.run_of, run_of.data(), run_of.size() * 4, hipMemcpyHostToDevice, s));
+    LQ_TRY(upload_descs(e));
+    drop_graphs(e);
+    return LQ_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t lq_engine_bytes_per_run(int64_t np, int64_t ncell) {
+    // 2 states (124 B) + 2 ini (96 B) + rates (56 B) + Shepard scratch (24 B)
+    // + run id (4 B) + sort keys/values (16 B) + histograms, + table 8 B/cell.
+    const int64_t per_particle = 2 * 124 + 2 * 96 + 56 + 24 + 4 + 16 + 1;
+    return np * per_particle + ncell * 8 + (int64_t)sizeof(RunDesc);
+}
+
+extern "C" lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t max_runs, lq_engine** out) {
+    if (!out || np_cap <= 0 || ncell_cap <= 0 || max_runs <= 0 || np_cap > 0x7fffffff || ncell_cap > 0x7fffffff) {
+        set_error("lq_engine_create: invalid capacity");
+        return LQ_ERR_INVALID_ARG;
+    }
+    lq_engine* e = new lq_engine();
+    e->np_cap = np_cap;
+    e->ncell_cap = ncell_cap;
+    e->max_runs = max_runs;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        set_error("hipStreamCreate failed");
+        return LQ_ERR_HIP;
+    }
+    Buffers& b = e->b;
+    bool ok = true;
+    for (int i = 0; i < 2 && ok; ++i) {
+        DevState& S = b.S[i];
+        ok = ok && dmalloc(S.xd, np_cap, e->owned) && dmalloc(S.yd, np_cap, e->owned) &&
+             dmalloc(S.zd, np_cap, e->owned) && dmalloc(S.pospres, np_cap, e->owned) &&
+             dmalloc(S.velrhop, np_cap, e->owned) && dmalloc(S.idep, np_cap, e->owned) &&
+             dmalloc(S.str, np_cap, e->owned) && dmalloc(S.stn, np_cap, e->owned) &&
+             dmalloc(S.orig, np_cap, e->owned);
+        DevIni& I = b.I[i];
+        ok = ok && dmalloc(I.xd, np_cap, e->owned) && dmalloc(I.yd, np_cap, e->owned) &&
+             dmalloc(I.zd, np_cap, e->owned) && dmalloc(I.velrhop, np_cap, e->owned) &&
+             dmalloc(I.epvr, np_cap, e->owned) && dmalloc(I.str, np_cap, e->owned) && dmalloc(I.stn, np_cap, e->owned);
+    }
+    ok = ok && dmalloc(b.R.accad, np_cap, e->owned) && dmalloc(b.R.spin, np_cap, e->owned) &&
+         dmalloc(b.R.acestn, np_cap, e->owned) && dmalloc(b.T, np_cap, e->owned) &&
+         dmalloc(b.run_of, np_cap, e->owned) && dmalloc(b.desc, max_runs, e->owned);
+    if (!ok || alloc_grid(b.grid, np_cap, ncell_cap) != LQ_OK) {
+        for (void* p : e->owned) (void)hipFree(p);
+        (void)hipStreamDestroy(e->stream);
+        delete e;
+        set_error("lq_engine_create: device allocation failed");
+        return LQ_ERR_OUT_OF_MEMORY;
+    }
+    *out = e;
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_engine_destroy(lq_engine* e) {
+    if (!e) return LQ_OK;
+    (void)hipStreamSynchronize(e->stream);
+    drop_graphs(e);
+    free_grid(e->b.grid);
+    for (void* p : e->owned) (void)hipFree(p);
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+    return LQ_OK;
+}
+
+extern "C" lq_stream lq_engine_stream(const lq_engine* e) { return e ? (lq_stream)e->stream : nullptr; }
+
+extern "C" lq_status lq_engine_set_graphs(lq_engine* e, int32_t use_graphs) {
+    if (!e) {
+        set_error("null engine");
+        return LQ_ERR_INVALID_ARG;
+    }
+    e->use_graphs = use_graphs != 0;
+    return LQ_OK;
+}
+
+static int64_t steps_until(double t, double tmax, double dt) {
+    int64_t n = 0;
+    while (t < tmax) {
+        t += dt;
+        ++n;
+    }
+    return n;
+}
+
+extern "C" lq_status lq_engine_add_run(lq_engine* e, const lq_run_input* in, int32_t* run_id) {
+    if (!e || !in || !run_id || !in->host.pospres || !in->host.velrhop || !in->host.idep || !in->host.str ||
+        !in->host.stn) {
+        set_error("lq_engine_add_run: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    lq_params par = in->par;
+    LQ_TRY(lq_params_derive(&par));
+    LQ_TRY(check_params(&par));
+    if (par.constitutive_model == 3 && !(in->vw.phi_c > 0.0f && in->vw.hs > 0.0f && in->vw.n > 0.0f)) {
+        set_error("constitutive model 3 needs phi_c, hs, n > 0");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if ((int64_t)e->runs.size() >= e->max_runs) {
+        set_error("engine run table is full");
+        return LQ_ERR_CAPACITY;
+    }
+    if (e->np_total + par.np > e->np_cap || e->ncell_total + par.ncell > e->ncell_cap) {
+        set_error("engine particle or cell capacity exceeded");
+        return LQ_ERR_CAPACITY;
+    }
+    const int np = par.np;
+    const float* pp = in->host.pospres;
+    std::vector<double> xd(np), yd(np), zd(np);
+    std::vector<float> pospres(pp, pp + 4 * (size_t)np);
+    for (int i = 0; i < np; ++i) {
+        if (in->posd) {
+            xd[i] = in->posd[3 * (size_t)i];
+            yd[i] = in->posd[3 * (size_t)i + 1];
+            zd[i] = in->posd[3 * (size_t)i + 2];
+            pospres[4 * (size_t)i] = (float)xd[i];
+            pospres[4 * (size_t)i + 1] = (float)yd[i];
+            pospres[4 * (size_t)i + 2] = (float)zd[i];
+        } else {
+            xd[i] = pp[4 * (size_t)i];
+            yd[i] = pp[4 * (size_t)i + 1];
+            zd[i] = pp[4 * (size_t)i + 2];
+        }
+    }
+    // Walls must stay inside the grid: the wall/material split is by index
+    // after each sort, and a wall sent to ncell_delete would land past the
+    // material block.
+    for (int i = 0; i < par.npb; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            const float v = pospres[4 * (size_t)i + a];
+            const int n = a == 0 ? par.cell_num_x : (a == 1 ? par.cell_num_y : par.cell_num_z);
+            const float f = floorf((0.5f * v) * par.i_h);
+            if (!(f >= 0.0f) || !(f < (float)n)) {
+                set_error("wall particle " + std::to_string(i) + " lies outside the cell grid");
+                return LQ_ERR_DOMAIN;
+            }
+        }
+    }
+    LQ_HIP_TRY(hipStreamSynchronize(e->stream));
+    RunSlot slot;
+    slot.id = e->next_id++;
+    fill_desc_static(slot.desc, par, &in->vw);
+    slot.desc.p_off = (int32_t)e->np_total;
+    slot.desc.cell_base = (int32_t)e->ncell_total;
+    slot.nsteps = in->nsteps > 0 ? in->nsteps : steps_until((double)par.time, (double)par.time_max, (double)par.dt);
+    slot.desc.nsteps = (int32_t)std::min<int64_t>(slot.nsteps, 0x7fffffff);
+    slot.desc.active = slot.desc.nsteps > 0 ? 1 : 0;
+    const DevState& S = e->b.S[e->cur];
+    const int64_t o = e->np_total;
+    std::vector<int32_t> orig(np), run_of(np, (int32_t)e->runs.size());
+    for (int i = 0; i < np; ++i) orig[i] = i;
+    if (np > 0) {
+        LQ_HIP_TRY(hipMemcpy(S.xd + o, xd.data(), np * 8, hipMemcpyHostToDevice));
+        LQ_HIP_TRY(hipMemcpy(S.yd + o, yd.data(), np * 8, hipMemcpyHostToDevice));
+        LQ_HIP_TRY(hipMemcpy(S.zd + o, zd.data(), np * 8, hipMemcpyHostToDevice));
+        LQ_HIP_TRY(hipMemcpy(S.pospres + o, pospres.data(), np * 16, hipMemcpyHostToDevice));
+        LQ_HIP_TRY(hipMemcpy(S.velrhop + o, in->host.velrhop, np * 16, hipMemcpyHostToDevice));
+        LQ_HIP_TRY(hipMemcpy(S.idep + o, in->host.idep, np * 16, hipMemcpyHostToDevice));
+        LQ_HIP_TRY(hipMemcpy(S.str + o, in->host.str, np * 24, hipMemcpyHostToDevice));
+        LQ_HIP_TRY(hipMemcpy(S.stn + o, in->host.stn, np * 24, hipMemcpyHostToDevice));
+        LQ_HIP_TRY(hipMemcpy(S.orig + o, orig.data(), np * 4, hipMemcpyHostToDevice));
+        LQ_HIP_TRY(hipMemcpy(e->b.run_of + o, run_of.data(), np * 4, hipMemcpyHostToDevice));
+    }
+    e->runs.push_back(slot);
+    e->np_total += np;
+    e->ncell_total += par.ncell;
+    LQ_TRY(upload_descs(e));
+    drop_graphs(e);
+    // Initial neighbour search + Primary sort (src/loquat.cu:297-298).  For
+    // the runs already resident this re-sorts by unchanged keys, which the
+    // next step's stable sort would do anyway.
+    Buffers& b = e->b;
+    enqueue_build(e, b.S[e->cur], e->stream);
+    launch_permute(b.grid.vals[b.grid.sorted_slot], (int)e->np_total, b.S[e->cur], b.S[1 - e->cur], nullptr, nullptr,
+                   nullptr, e->stream);
+    e->cur = 1 - e->cur;
+    LQ_TRY(last_launch("lq_engine_add_run"));
+    LQ_HIP_TRY(hipStreamSynchronize(e->stream));
+    *run_id = slot.id;
+    return LQ_OK;
+}
+
+static int find_run(const lq_engine* e, int32_t id) {
+    for (size_t i = 0; i < e->runs.size(); ++i)
+        if (e->runs[i].id == id) return (int)i;
+    return -1;
+}
+
+extern "C" lq_status lq_engine_remove_run(lq_engine* e, int32_t run_id) {
+    if (!e) {
+        set_error("null engine");
+        return LQ_ERR_INVALID_ARG;
+    }
+    const int i = find_run(e, run_id);
+    if (i < 0) {
+        set_error("unknown run id");
+        return LQ_ERR_INVALID_ARG;
+    }
+    e->runs.erase(e->runs.begin() + i);
+    return relayout(e);
+}
+
+extern "C" lq_status lq_engine_step(lq_engine* e, int32_t nsteps) {
+    if (!e || nsteps < 0) {
+        set_error("lq_engine_step: invalid argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    for (int it = 0; it < nsteps; ++it) {
+        bool any_active = false;
+        for (const RunSlot& r : e->runs) any_active = any_active || r.desc.active;
+        if (!any_active) break;
+        const bool shep = any_shepard_due(e);
+        if (e->use_graphs) {
+            hipGraphExec_t& g = e->graph[e->cur][shep ? 1 : 0];
+            if (!g) {
+                hipGraph_t graph = nullptr;
+                LQ_HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+                enqueue_step(e, e->cur, shep, nullptr);
+                LQ_HIP_TRY(hipStreamEndCapture(e->stream, &graph));
+                hipError_t err = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (err != hipSuccess) {
+                    g = nullptr;
+                    set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(err));
+                    return LQ_ERR_HIP;
+                }
+            }
+            LQ_HIP_TRY(hipGraphLaunch(g, e->stream));
+        } else {
+            enqueue_step(e, e->cur, shep, nullptr);
+            LQ_TRY(last_launch("lq_engine_step"));
+        }
+        if (shep) e->cur = 1 - e->cur;
+        host_step_end(e);
+    }
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_engine_profile(lq_engine* e, int32_t nsteps, double* ms_stage5) {
+    if (!e || nsteps <= 0 || !ms_stage5) {
+        set_error("lq_engine_profile: invalid argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    Prof p;
+    std::memset(&p, 0, sizeof(p));
+    LQ_HIP_TRY(hipEventCreate(&p.ev[0]));
+    LQ_HIP_TRY(hipEventCreate(&p.ev[1]));
+    int done = 0;
+    for (int it = 0; it < nsteps; ++it) {
+        bool any_active = false;
+        for (const RunSlot& r : e->runs) any_active = any_active || r.desc.active;
+        if (!any_active) break;
+        const bool shep = any_shepard_due(e);
+        enqueue_step(e, e->cur, shep, &p);
+        if (shep) e->cur = 1 - e->cur;
+        host_step_end(e);
+        ++done;
+    }
+    (void)hipEventDestroy(p.ev[0]);
+    (void)hipEventDestroy(p.ev[1]);
+    LQ_TRY(last_launch("lq_engine_profile"));
+    for (int i = 0; i < 5; ++i) ms_stage5[i] = done ? p.ms[i] / done : 0.0;
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_engine_synchronize(lq_engine* e) {
+    if (!e) {
+        set_error("null engine");
+        return LQ_ERR_INVALID_ARG;
+    }
+    LQ_HIP_TRY(hipStreamSynchronize(e->stream));
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_engine_run_info(const lq_engine* e, int32_t run_id, int64_t* steps_done, double* time,
+                                        int32_t* finished, int32_t* np) {
+    if (!e) {
+        set_error("null engine");
+        return LQ_ERR_INVALID_ARG;
+    }
+    const int i = find_run(e, run_id);
+    if (i < 0) {
+        set_error("unknown run id");
+        return LQ_ERR_INVALID_ARG;
+    }
+    const RunDesc& d = e->runs[i].desc;
+    if (steps_done) *steps_done = d.step;
+    if (time) *time = d.time;
+    if (finished) *finished = d.active ? 0 : 1;
+    if (np) *np = d.np;
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_engine_totals(const lq_engine* e, int64_t* npm, int64_t* np, int64_t* ncell, int32_t* nruns) {
+    if (!e) {
+        set_error("null engine");
+        return LQ_ERR_INVALID_ARG;
+    }
+    int64_t m = 0;
+    for (const RunSlot& r : e->runs) m += r.desc.npm;
+    if (npm) *npm = m;
+    if (np) *np = e->np_total;
+    if (ncell) *ncell = e->ncell_total;
+    if (nruns) *nruns = (int32_t)e->runs.size();
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_engine_download(lq_engine* e, int32_t run_id, lq_host_particles* out, double* posd) {
+    if (!e || !out) {
+        set_error("lq_engine_download: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    const int i = find_run(e, run_id);
+    if (i < 0) {
+        set_error("unknown run id");
+        return LQ_ERR_INVALID_ARG;
+    }
+    LQ_HIP_TRY(hipStreamSynchronize(e->stream));
+    const RunDesc& d = e->runs[i].desc;
+    const int np = d.np;
+    const int64_t o = d.p_off;
+    const DevState& S = e->b.S[e->cur];
+    std::vector<double> xd(np), yd(np), zd(np);
+    std::vector<float4> pp(np), vr(np), id(np);
+    std::vector<tensor2d> str(np), stn(np);
+    std::vector<int32_t> orig(np);
+    if (np > 0) {
+        LQ_HIP_TRY(hipMemcpy(xd.data(), S.xd + o, np * 8, hipMemcpyDeviceToHost));
+        LQ_HIP_TRY(hipMemcpy(yd.data(), S.yd + o, np * 8, hipMemcpyDeviceToHost));
+        LQ_HIP_TRY(hipMemcpy(zd.data(), S.zd + o, np * 8, hipMemcpyDeviceToHost));
+        LQ_HIP_TRY(hipMemcpy(pp.data(), S.pospres + o, np * 16, hipMemcpyDeviceToHost));
+        LQ_HIP_TRY(hipMemcpy(vr.data(), S.velrhop + o, np * 16, hipMemcpyDeviceToHost));
+        LQ_HIP_TRY(hipMemcpy(id.data(), S.idep + o, np * 16, hipMemcpyDeviceToHost));
+        LQ_HIP_TRY(hipMemcpy(str.data(), S.str + o, np * 24, hipMemcpyDeviceToHost));
+        LQ_HIP_TRY(hipMemcpy(stn.data(), S.stn + o, np * 24, hipMemcpyDeviceToHost));
+        LQ_HIP_TRY(hipMemcpy(orig.data(), S.orig + o, np * 4, hipMemcpyDeviceToHost));
+    }
+    for (int k = 0; k < np; ++k) {
+        const int t = orig[k];
+        if (t < 0 || t >= np) {
+            set_error("corrupt original-index array");
+            return LQ_ERR_STATE;
+        }
+        if (out->pospres) std::memcpy(out->pospres + 4 * (size_t)t, &pp[k], 16);
+        if (out->velrhop) std::memcpy(out->velrhop + 4 * (size_t)t, &vr[k], 16);
+        if (out->idep) std::memcpy(out->idep + 4 * (size_t)t, &id[k], 16);
+        if (out->str) std::memcpy(out->str + 6 * (size_t)t, &str[k], 24);
+        if (out->stn) std::memcpy(out->stn + 6 * (size_t)t, &stn[k], 24);
+        if (posd) {
+            posd[3 * (size_t)t] = xd[k];
+            posd[3 * (size_t)t + 1] = yd[k];
+            posd[3 * (size_t)t + 2] = zd[k];
+        }
+    }
+    return LQ_OK;
+}

@@ -1,0 +1,329 @@
+// lq_grid.hip -- neighbour grid for gfx950: cell hash, stable LSD radix sort,
+// cell begin/end table and the state permute.
+//
+// Replaces LoquatGrid (src/loquat_grid.h:28-66), whose body ships only as a
+// prebuilt library, and the cub::DeviceRadixSort::SortPairs call it is built
+// around (stable ascending sort of (cell, index) pairs,
+// include/cub/device/device_radix_sort.cuh:147-175).  The result is the unique
+// stable order, so it matches any stable sort of the same keys bit for bit.
+//
+// Radix sort, per 8-bit digit pass (ceil(bits/8) passes, bits = ceil(log2
+// ncell)):
+//   count   : one workgroup per 4096-key tile; each wave ranks 64 keys per
+//             round with 8 ballots (wave-wide match of the digit), the lowest
+//             lane of each peer group bumps the wave's LDS counter -- no LDS
+//             atomics, so runs of equal digits (sorted particles share cells)
+//             cost nothing extra.
+//   scan    : one workgroup per digit scans that digit's tile counts.
+//   scatter : re-ranks the tile the same way, stages keys/values in LDS in
+//             digit order and writes each digit's run contiguously.
+#include "lq_internal.hpp"
+
+namespace lq {
+
+namespace {
+
+__device__ __forceinline__ unsigned long long match8(uint32_t d, bool valid) {
+    const unsigned long long vm = __ballot(valid);
+    unsigned long long peers = vm;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const unsigned long long m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers & vm;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan over the 256 threads of a workgroup; total of all values out.
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* sm4, uint32_t& total) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) sm4[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t x = sm4[i];
+        if (i < w) pre += x;
+        tot += x;
+    }
+    total = tot;
+    __syncthreads();
+    return pre + inc - v;
+}
+
+constexpr int WAVE_CHUNK = RS_TILE / (RS_THREADS / 64);  // keys per wave per tile
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_count(const uint32_t* __restrict__ keys, int n, int shift,
+                                                             uint32_t* __restrict__ hist, int ntiles) {
+    __shared__ uint32_t cnt[4][RS_BINS];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    for (int i = t; i < 4 * RS_BINS; i += RS_THREADS) (&cnt[0][0])[i] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE + w * WAVE_CHUNK;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+#pragma unroll 4
+    for (int r = 0; r < RS_IPT; ++r) {
+        const int idx = base + r * 64 + lane;
+        const bool valid = idx < n;
+        const uint32_t d = valid ? (keys[idx] >> shift) & 255u : 0u;
+        const unsigned long long peers = match8(d, valid);
+        if (valid && (peers & lt) == 0ull) cnt[w][d] += (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    hist[(size_t)t * ntiles + blockIdx.x] = cnt[0][t] + cnt[1][t] + cnt[2][t] + cnt[3][t];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_scan(uint32_t* __restrict__ hist, int ntiles,
+                                                            uint32_t* __restrict__ totals) {
+    __shared__ uint32_t sm[4];
+    uint32_t* row = hist + (size_t)blockIdx.x * ntiles;
+    uint32_t running = 0;
+    for (int c = 0; c < ntiles; c += RS_THREADS) {
+        const int i = c + threadIdx.x;
+        const uint32_t v = i < ntiles ? row[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan_256(v, sm, tot);
+        if (i < ntiles) row[i] = ex + running;
+        running += tot;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = running;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                               const uint32_t* __restrict__ vin,
+                                                               uint32_t* __restrict__ kout,
+                                                               uint32_t* __restrict__ vout, int n, int shift,
+                                                               const uint32_t* __restrict__ hist,
+                                                               const uint32_t* __restrict__ totals, int ntiles) {
+    __shared__ uint32_t cnt[4][RS_BINS];
+    __shared__ uint32_t lbase[RS_BINS];
+    __shared__ uint32_t goff[RS_BINS];
+    __shared__ uint32_t sk[RS_TILE];
+    __shared__ uint32_t sv[RS_TILE];
+    __shared__ uint32_t sm[4];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    for (int i = t; i < 4 * RS_BINS; i += RS_THREADS) (&cnt[0][0])[i] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE + w * WAVE_CHUNK;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    uint32_t key[RS_IPT], val[RS_IPT], rank[RS_IPT];
+#pragma unroll
+    for (int r = 0; r < RS_IPT; ++r) {
+        const int idx = base + r * 64 + lane;
+        const bool valid = idx < n;
+        key[r] = valid ? kin[idx] : 0u;
+        val[r] = valid ? (vin ? vin[idx] : (uint32_t)idx) : 0u;
+        const uint32_t d = (key[r] >> shift) & 255u;
+        const unsigned long long peers = match8(d, valid);
+        rank[r] = 0;
+        if (valid) {
+            const uint32_t b = cnt[w][d];
+            const unsigned long long below = peers & lt;
+            rank[r] = b + (uint32_t)__popcll(below);
+            if (below == 0ull) cnt[w][d] = b + (uint32_t)__popcll(peers);
+        }
+    }
+    __syncthreads();
+    uint32_t run = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t c = cnt[i][t];
+        cnt[i][t] = run;
+        run += c;
+    }
+    uint32_t tot_local, tot_global;
+    const uint32_t ex = block_excl_scan_256(run, sm, tot_local);
+    lbase[t] = ex;
+    const uint32_t gex = block_excl_scan_256(totals[t], sm, tot_global);
+    goff[t] = gex + hist[(size_t)t * ntiles + blockIdx.x] - ex;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_IPT; ++r) {
+        const int idx = base + r * 64 + lane;
+        if (idx < n) {
+            const uint32_t d = (key[r] >> shift) & 255u;
+            const uint32_t pos = lbase[d] + cnt[w][d] + rank[r];
+            sk[pos] = key[r];
+            sv[pos] = val[r];
+        }
+    }
+    __syncthreads();
+    const int nvalid = min(RS_TILE, n - (int)blockIdx.x * RS_TILE);
+    for (int p = t; p < nvalid; p += RS_THREADS) {
+        const uint32_t k = sk[p];
+        const uint32_t g = goff[(k >> shift) & 255u] + (uint32_t)p;
+        kout[g] = k;
+        vout[g] = sv[p];
+    }
+}
+
+// Cell id of every particle: layer 0 for walls, 1 for material (index rule of
+// src/adami_boundary.cu:73 and src/loquat.cu:119-124); any axis outside the
+// grid sends the particle to ncell_delete = ncell - 1.
+template <class DP>
+__global__ __launch_bounds__(256) void k_hash(DP dp, const float4* __restrict__ pospres, int n,
+                                              uint32_t* __restrict__ keys) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const RunDesc& d = dp.get(dp.run(k));
+    const int layer = (k - d.p_off) < d.npb ? 0 : 1;
+    const float4 p = pospres[k];
+    const float i_h = d.par.i_h;
+    const int cx = cell_coord(p.x, i_h), cy = cell_coord(p.y, i_h), cz = cell_coord(p.z, i_h);
+    const int nx = d.par.cell_num_x, ny = d.par.cell_num_y, nz = d.par.cell_num_z;
+    uint32_t key;
+    if (cx < 0 || cx >= nx || cy < 0 || cy >= ny || cz < 0 || cz >= nz)
+        key = (uint32_t)(d.cell_base + d.par.ncell - 1);
+    else
+        key = (uint32_t)(d.cell_base + layer * d.par.ncell_one_layer + (cx * ny + cy) * nz + cz);
+    keys[k] = key;
+}
+
+__global__ void k_fill2(int32_t* __restrict__ a, int32_t* __restrict__ b, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        a[i] = v;
+        b[i] = v;
+    }
+}
+
+__global__ void k_cell_table(const uint32_t* __restrict__ keys, int n, int32_t* __restrict__ beg,
+                             int32_t* __restrict__ end) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t key = keys[k];
+    if (k == 0 || keys[k - 1] != key) beg[key] = k;
+    if (k == n - 1 || keys[k + 1] != key) end[key] = k + 1;
+}
+
+__global__ __launch_bounds__(256) void k_permute(const uint32_t* __restrict__ perm, int n, DevState src, DevState dst,
+                                                 int with_ini, DevIni isrc, DevIni idst, tensor2d* str_dst) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int s = (int)perm[k];
+    dst.xd[k] = src.xd[s];
+    dst.yd[k] = src.yd[s];
+    dst.zd[k] = src.zd[s];
+    dst.pospres[k] = src.pospres[s];
+    dst.velrhop[k] = src.velrhop[s];
+    dst.idep[k] = src.idep[s];
+    (str_dst ? str_dst : dst.str)[k] = src.str[s];
+    dst.stn[k] = src.stn[s];
+    dst.orig[k] = src.orig[s];
+    if (with_ini) {
+        idst.xd[k] = isrc.xd[s];
+        idst.yd[k] = isrc.yd[s];
+        idst.zd[k] = isrc.zd[s];
+        idst.velrhop[k] = isrc.velrhop[s];
+        idst.epvr[k] = isrc.epvr[s];
+        idst.str[k] = isrc.str[s];
+        idst.stn[k] = isrc.stn[s];
+    }
+}
+
+inline int blocks_for(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+lq_status alloc_grid(GridBuffers& g, int64_t np_cap, int64_t ncell_cap) {
+    g = GridBuffers{};
+    g.np_cap = np_cap;
+    g.ncell_cap = ncell_cap;
+    const int64_t np_alloc = np_cap > 0 ? np_cap : 1;
+    const int64_t nc_alloc = ncell_cap > 0 ? ncell_cap : 1;
+    const int64_t tiles = rs_tiles((int)np_alloc);
+    void* p = nullptr;
+    for (int i = 0; i < 2; ++i) {
+        if (hipMalloc(&p, np_alloc * 4) != hipSuccess) goto oom;
+        g.keys[i] = (uint32_t*)p;
+        if (hipMalloc(&p, np_alloc * 4) != hipSuccess) goto oom;
+        g.vals[i] = (uint32_t*)p;
+    }
+    if (hipMalloc(&p, RS_BINS * tiles * 4) != hipSuccess) goto oom;
+    g.hist = (uint32_t*)p;
+    if (hipMalloc(&p, RS_BINS * 4) != hipSuccess) goto oom;
+    g.totals = (uint32_t*)p;
+    if (hipMalloc(&p, nc_alloc * 4) != hipSuccess) goto oom;
+    g.cell_beg = (int32_t*)p;
+    if (hipMalloc(&p, nc_alloc * 4) != hipSuccess) goto oom;
+    g.cell_end = (int32_t*)p;
+    g.sorted_slot = -1;
+    return LQ_OK;
+oom:
+    free_grid(g);
+    set_error("hipMalloc failed for the neighbour grid");
+    return LQ_ERR_OUT_OF_MEMORY;
+}
+
+void free_grid(GridBuffers& g) {
+    for (int i = 0; i < 2; ++i) {
+        if (g.keys[i]) (void)hipFree(g.keys[i]);
+        if (g.vals[i]) (void)hipFree(g.vals[i]);
+        g.keys[i] = g.vals[i] = nullptr;
+    }
+    if (g.hist) (void)hipFree(g.hist);
+    if (g.totals) (void)hipFree(g.totals);
+    if (g.cell_beg) (void)hipFree(g.cell_beg);
+    if (g.cell_end) (void)hipFree(g.cell_end);
+    g.hist = g.totals = nullptr;
+    g.cell_beg = g.cell_end = nullptr;
+}
+
+void launch_hash_one(const RunDesc& d, const float4* pospres, int n, uint32_t* keys, hipStream_t s) {
+    if (n <= 0) return;
+    k_hash<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, pospres, n, keys);
+}
+
+void launch_hash_table(const RunDesc* d, const int32_t* run_of, const float4* pospres, int n, uint32_t* keys,
+                       hipStream_t s) {
+    if (n <= 0) return;
+    k_hash<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, pospres, n, keys);
+}
+
+void launch_radix_sort(GridBuffers& g, int n, int bits, hipStream_t s) {
+    if (n <= 0) {
+        g.sorted_slot = 0;
+        return;
+    }
+    const int ntiles = rs_tiles(n);
+    const int passes = (bits + 7) / 8;
+    int src = 0;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        k_radix_count<<<ntiles, RS_THREADS, 0, s>>>(g.keys[src], n, shift, g.hist, ntiles);
+        k_radix_scan<<<RS_BINS, RS_THREADS, 0, s>>>(g.hist, ntiles, g.totals);
+        k_radix_scatter<<<ntiles, RS_THREADS, 0, s>>>(g.keys[src], p == 0 ? nullptr : g.vals[src],
+                                                       g.keys[1 - src], g.vals[1 - src], n, shift, g.hist,
+                                                       g.totals, ntiles);
+        src = 1 - src;
+    }
+    g.sorted_slot = src;
+}
+
+void launch_cell_table(GridBuffers& g, int n, int64_t ncell_total, hipStream_t s) {
+    const int fill_blocks = (int)std::min<int64_t>(blocks_for(ncell_total, 256), 4096);
+    if (ncell_total > 0) k_fill2<<<fill_blocks, 256, 0, s>>>(g.cell_beg, g.cell_end, ncell_total, -1);
+    if (n > 0) k_cell_table<<<blocks_for(n, 256), 256, 0, s>>>(g.keys[g.sorted_slot], n, g.cell_beg, g.cell_end);
+}
+
+void launch_permute(const uint32_t* perm, int n, const DevState& src, const DevState& dst, const DevIni* ini_src,
+                    const DevIni* ini_dst, tensor2d* str_dst, hipStream_t s) {
+    if (n <= 0) return;
+    const int with_ini = (ini_src && ini_dst) ? 1 : 0;
+    DevIni a = with_ini ? *ini_src : DevIni{};
+    DevIni b = with_ini ? *ini_dst : DevIni{};
+    k_permute<<<blocks_for(n, 256), 256, 0, s>>>(perm, n, src, dst, with_ini, a, b, str_dst);
+}
+
+}  // namespace lq

@@ -1,0 +1,106 @@
+// lq_internal.hpp -- host-side launchers shared between the module ABI and
+// the engine.  Every launcher is asynchronous on the given stream, allocates
+// nothing and records no memset (so it can be captured into a hipGraph).
+#pragma once
+
+#include <algorithm>
+#include <string>
+
+#include "lq_common.hpp"
+
+namespace lq {
+
+// Thread-local error text for lq_last_error().
+void set_error(const std::string& msg);
+#define LQ_HIP_TRY(expr)                                                                 \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            ::lq::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));          \
+            return LQ_ERR_HIP;                                                           \
+        }                                                                                \
+    } while (0)
+#define LQ_TRY(expr)                      \
+    do {                                  \
+        lq_status s_ = (expr);            \
+        if (s_ != LQ_OK) return s_;       \
+    } while (0)
+
+// Host-side derivations.
+float neighbour_r2_cut(float i_h);
+void fill_desc_static(RunDesc& d, const lq_params& par, const lq_vw_params* vw);
+
+// ---------------------------------------------------------------- grid
+// Radix-sort geometry.
+constexpr int RS_THREADS = 256;
+constexpr int RS_IPT = 16;
+constexpr int RS_TILE = RS_THREADS * RS_IPT;  // 4096 keys per tile
+constexpr int RS_BINS = 256;
+
+inline int rs_tiles(int n) { return (n + RS_TILE - 1) / RS_TILE; }
+inline int rs_bits(int64_t ncell) {
+    int b = 1;
+    while ((int64_t(1) << b) < ncell) ++b;
+    return b;
+}
+
+// Scratch owned by a grid: keys/values ping-pong, histograms, cell table.
+struct GridBuffers {
+    uint32_t* keys[2];
+    uint32_t* vals[2];
+    uint32_t* hist;    // RS_BINS * tiles
+    uint32_t* totals;  // RS_BINS
+    int32_t* cell_beg;
+    int32_t* cell_end;
+    int64_t np_cap;
+    int64_t ncell_cap;
+    int sorted_slot;  // which keys/vals slot holds the sorted result
+};
+
+lq_status alloc_grid(GridBuffers& g, int64_t np_cap, int64_t ncell_cap);
+void free_grid(GridBuffers& g);
+
+// Cell hash of N particles (one run by value, or a table).
+void launch_hash_one(const RunDesc& d, const float4* pospres, int n, uint32_t* keys, hipStream_t s);
+void launch_hash_table(const RunDesc* d, const int32_t* run_of, const float4* pospres, int n, uint32_t* keys,
+                       hipStream_t s);
+// Stable LSD radix sort of keys[0] with values = iota; result in slot g.sorted_slot.
+void launch_radix_sort(GridBuffers& g, int n, int bits, hipStream_t s);
+// cell_beg / cell_end of the sorted keys (-1 for empty cells).
+void launch_cell_table(GridBuffers& g, int n, int64_t ncell_total, hipStream_t s);
+// dst[k] = src[perm[k]] for all state arrays (and ini when both set).  When
+// str_dst is non-null the permuted stress goes there instead of dst.str.
+void launch_permute(const uint32_t* perm, int n, const DevState& src, const DevState& dst, const DevIni* ini_src,
+                    const DevIni* ini_dst, tensor2d* str_dst, hipStream_t s);
+
+// ---------------------------------------------------------------- physics
+enum Stage { ST_ADAMI = 0, ST_INTERACT = 1, ST_PREDICT = 2, ST_CORRECT = 3, ST_SHEPARD = 4 };
+
+void launch_adami_one(const RunDesc& d, const DevState& st, const int32_t* beg, const int32_t* end, int n,
+                      hipStream_t s);
+void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const int32_t* beg,
+                        const int32_t* end, int n, hipStream_t s);
+void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates& rt, const int32_t* beg,
+                            const int32_t* end, int n, int32_t* nbr, int variant, hipStream_t s);
+void launch_interaction_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt,
+                              const int32_t* beg, const int32_t* end, int n, int32_t* nbr, int variant,
+                              hipStream_t s);
+void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, const DevIni& ini,
+                          const DevRates& rt, int n, hipStream_t s);
+void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
+                            const DevIni& ini, const DevRates& rt, int n, hipStream_t s);
+// Shepard filter: reads stress from str_src, writes str_dst (walls and runs
+// not due are copied through).
+void launch_shepard_one(const RunDesc& d, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,
+                        const int32_t* beg, const int32_t* end, int n, hipStream_t s);
+void launch_shepard_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const tensor2d* str_src,
+                          tensor2d* str_dst, const int32_t* beg, const int32_t* end, int n, hipStream_t s);
+// Per-run step bookkeeping on the device.
+void launch_step_begin(RunDesc* d, int nruns, hipStream_t s);
+void launch_step_end(RunDesc* d, int nruns, hipStream_t s);
+// Copies material stress back (reference's D2D copy, src/field_smoothing.cu:144).
+void launch_copy_tensor(const tensor2d* src, tensor2d* dst, int n, hipStream_t s);
+
+int interaction_variant();
+
+}  // namespace lq

@@ -1,0 +1,718 @@
+// lq_physics.hip -- per-particle SPH stages for gfx950.
+//
+//   Adami wall stress         src/adami_boundary.cu:27-123
+//   particle interaction      src/particle_interaction.h:29-41 (body not shipped;
+//                             formulation fixed by its consumers,
+//                             src/predictor_corrector.cu:186-211,276-300)
+//   predictor / corrector     src/predictor_corrector.cu:169-218, 256-307
+//   Drucker-Prager            src/predictor_corrector.cu:30-105
+//   Wu hypoplastic            src/predictor_corrector.cu:111-165
+//   von Wolffersdorff         constitutive model 3, no reference counterpart
+//   Shepard stress filter     src/field_smoothing.cu:27-121
+//
+// One thread per particle of the packed multi-run layout.  Neighbour loops
+// walk the nine (x, y) columns of the 3x3x3 cell stencil per layer; the
+// neighbour test uses the FP32 mirror pospres (bit-identical predicate on
+// every implementation), pair terms use the FP64 position difference rounded
+// to FP32.  All sums and stress rates are FP32; positions and time are FP64;
+// the Drucker-Prager return mapping is FP64.
+#include "lq_internal.hpp"
+
+namespace lq {
+
+namespace {
+
+inline int blocks_for(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+__device__ __forceinline__ tensor2d tadd(const tensor2d& a, const tensor2d& b) {
+    return tensor2d{a.xx + b.xx, a.yy + b.yy, a.zz + b.zz, a.xy + b.xy, a.xz + b.xz, a.yz + b.yz};
+}
+__device__ __forceinline__ tensor2d tscale(const tensor2d& a, float s) {
+    return tensor2d{a.xx * s, a.yy * s, a.zz * s, a.xy * s, a.xz * s, a.yz * s};
+}
+// Double contraction with off-diagonals weighted twice (J2 of include/types.h:77-85).
+__device__ __forceinline__ float tdot(const tensor2d& a, const tensor2d& b) {
+    return a.xx * b.xx + a.yy * b.yy + a.zz * b.zz + 2.0f * (a.xy * b.xy + a.xz * b.xz + a.yz * b.yz);
+}
+
+// Jaumann co-rotational term W.sigma - sigma.W with spin (W_yx, W_zx, W_zy)
+// (src/predictor_corrector.cu:39-45 and :118-124).
+__device__ __forceinline__ tensor2d jaumann(const float4 w, const tensor2d& s) {
+    tensor2d r;
+    r.xx = -2.0f * (w.x * s.xy + w.y * s.xz);
+    r.yy = 2.0f * (w.x * s.xy - w.z * s.yz);
+    r.zz = 2.0f * (w.y * s.xz + w.z * s.yz);
+    r.xy = w.x * (s.xx - s.yy) - w.z * s.xz - w.y * s.yz;
+    r.xz = w.y * (s.xx - s.zz) + w.z * s.xy - w.x * s.yz;
+    r.yz = w.z * (s.yy - s.zz) + w.x * s.xz + w.y * s.xy;
+    return r;
+}
+
+// ------------------------------------------------------------ Adami walls
+template <class DP>
+__global__ __launch_bounds__(256) void k_adami(DP dp, DevState st, const int32_t* __restrict__ beg,
+                                               const int32_t* __restrict__ end, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const RunDesc& d = dp.get(dp.run(k));
+    if ((k - d.p_off) >= d.npb || !d.active) return;
+    const float4 pi = st.pospres[k];
+    const double xi = st.xd[k], yi = st.yd[k], zi = st.zd[k];
+    const float i_h = d.par.i_h, cw = d.par.kernel_normalization_par, m = d.par.m, r2cut = d.r2_cut;
+    float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
+    float px = 0.f, py = 0.f, pz = 0.f, w = 0.f;
+    const int cix = cell_coord(pi.x, i_h), ciy = cell_coord(pi.y, i_h), ciz = cell_coord(pi.z, i_h);
+    for_each_column(d, cix, ciy, ciz, 1, 1, beg, end, [&](int jb, int je) {
+        for (int j = jb; j < je; ++j) {
+            const float4 pj = st.pospres[j];
+            if (r2_fp32(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < r2cut) {
+                const float dx = (float)(xi - st.xd[j]), dy = (float)(yi - st.yd[j]), dz = (float)(zi - st.zd[j]);
+                const float q = fsqrt(dx * dx + dy * dy + dz * dz) * i_h;
+                const float t = fmaxf(2.0f - q, 0.0f);
+                const float t2 = t * t;
+                const float kv = cw * t2 * t2 * (q + 0.5f) * m * frcp(st.velrhop[j].w);
+                const tensor2d s = st.str[j];
+                sxx += s.xx * kv;
+                syy += s.yy * kv;
+                szz += s.zz * kv;
+                sxy += s.xy * kv;
+                sxz += s.xz * kv;
+                syz += s.yz * kv;
+                px += dx * kv;
+                py += dy * kv;
+                pz += dz * kv;
+                w += kv;
+            }
+        }
+    });
+    if (w > 0.0f) {
+        const float iw = 1.0f / w;
+        const float rho0 = d.par.rho0;
+        tensor2d o;
+        o.xx = (sxx - rho0 * d.par.acc_x * px) * iw;
+        o.yy = (syy - rho0 * d.par.acc_y * py) * iw;
+        o.zz = (szz - rho0 * d.par.acc_z * pz) * iw;
+        o.xy = sxy * iw;
+        o.xz = sxz * iw;
+        o.yz = syz * iw;
+        st.str[k] = o;
+    }
+}
+
+// ------------------------------------------------------------ interaction
+struct IPart {
+    double x, y, z;
+    float vx, vy, vz, rho;
+    tensor2d T;  // sigma_i / rho_i^2
+};
+struct IAcc {
+    float ax, ay, az, dvg;
+    float L[9];  // sum_j (v_j - v_i)_a / rho_j * dW_b  (times m at the end)
+};
+struct IConst {
+    float i_h, cg, h, eta, alpha_cs;
+};
+
+// One pair i-j (Appendix formulation: Wendland C2 gradient, SPH velocity
+// gradient, continuity, momentum with sigma_i/rho_i^2 + sigma_j/rho_j^2 and
+// Monaghan viscosity acting as a pressure).
+__device__ __forceinline__ void pair_eval(const IPart& pi, int j, const DevState& st, const IConst& c, IAcc& a) {
+    const float dx = (float)(pi.x - st.xd[j]), dy = (float)(pi.y - st.yd[j]), dz = (float)(pi.z - st.zd[j]);
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float q = fsqrt(r2) * c.i_h;
+    const float t = fmaxf(2.0f - q, 0.0f);
+    const float fac = c.cg * t * t * t;
+    const float gx = fac * dx, gy = fac * dy, gz = fac * dz;
+    const float4 vr = st.velrhop[j];
+    const tensor2d sj = st.str[j];
+    const float ir = frcp(vr.w);
+    const float dvx = vr.x - pi.vx, dvy = vr.y - pi.vy, dvz = vr.z - pi.vz;
+    const float ux = dvx * ir, uy = dvy * ir, uz = dvz * ir;
+    a.L[0] += ux * gx;
+    a.L[1] += ux * gy;
+    a.L[2] += ux * gz;
+    a.L[3] += uy * gx;
+    a.L[4] += uy * gy;
+    a.L[5] += uy * gz;
+    a.L[6] += uz * gx;
+    a.L[7] += uz * gy;
+    a.L[8] += uz * gz;
+    a.dvg += dvx * gx + dvy * gy + dvz * gz;
+    const float vdx = -(dvx * dx + dvy * dy + dvz * dz);  // (v_i - v_j).(x_i - x_j)
+    float pij = 0.0f;
+    if (vdx < 0.0f) {
+        const float mu = c.h * vdx * frcp(r2 + c.eta);
+        pij = -2.0f * c.alpha_cs * mu * frcp(pi.rho + vr.w);
+    }
+    const float ir2 = ir * ir;
+    const float sxx = pi.T.xx + sj.xx * ir2 - pij;
+    const float syy = pi.T.yy + sj.yy * ir2 - pij;
+    const float szz = pi.T.zz + sj.zz * ir2 - pij;
+    const float sxy = pi.T.xy + sj.xy * ir2;
+    const float sxz = pi.T.xz + sj.xz * ir2;
+    const float syz = pi.T.yz + sj.yz * ir2;
+    a.ax += sxx * gx + sxy * gy + sxz * gz;
+    a.ay += sxy * gx + syy * gy + syz * gz;
+    a.az += sxz * gx + syz * gy + szz * gz;
+}
+
+__device__ __forceinline__ void load_part(const DevState& st, int k, IPart& p) {
+    p.x = st.xd[k];
+    p.y = st.yd[k];
+    p.z = st.zd[k];
+    const float4 vr = st.velrhop[k];
+    p.vx = vr.x;
+    p.vy = vr.y;
+    p.vz = vr.z;
+    p.rho = vr.w;
+    const float ir = 1.0f / vr.w;
+    p.T = tscale(st.str[k], ir * ir);
+}
+
+__device__ __forceinline__ IConst make_iconst(const RunDesc& d) {
+    IConst c;
+    c.i_h = d.par.i_h;
+    c.cg = d.par.kernel_gradient_normalization_par;
+    c.h = d.par.h;
+    c.eta = d.par.eta;
+    c.alpha_cs = d.par.viscosity_type == 1 ? d.par.artificial_viscosity_alpha * d.par.cs : 0.0f;
+    return c;
+}
+
+__device__ __forceinline__ void store_rates(const RunDesc& d, const DevRates& rt, int k, const IAcc& a) {
+    const float m = d.par.m;
+    rt.accad[k] = make_float4(m * a.ax + d.par.acc_x, m * a.ay + d.par.acc_y, m * a.az + d.par.acc_z, -m * a.dvg);
+    const float hm = 0.5f * m;
+    // spin = (W_yx, W_zx, W_zy): sign fixed by the Jaumann term of
+    // src/predictor_corrector.cu:40-45 (rigid rotation about z gives spin.x = Omega).
+    rt.spin[k] = make_float4(hm * (a.L[3] - a.L[1]), hm * (a.L[6] - a.L[2]), hm * (a.L[7] - a.L[5]), 0.0f);
+    tensor2d e;
+    e.xx = m * a.L[0];
+    e.yy = m * a.L[4];
+    e.zz = m * a.L[8];
+    e.xy = hm * (a.L[1] + a.L[3]);
+    e.xz = hm * (a.L[2] + a.L[6]);
+    e.yz = hm * (a.L[5] + a.L[7]);
+    rt.acestn[k] = e;
+}
+
+// Variant 0: each thread walks its own candidates and evaluates hits inline.
+template <class DP>
+__global__ __launch_bounds__(256) void k_interaction_direct(DP dp, DevState st, DevRates rt,
+                                                            const int32_t* __restrict__ beg,
+                                                            const int32_t* __restrict__ end, int n,
+                                                            int32_t* __restrict__ nbr) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const RunDesc& d = dp.get(dp.run(k));
+    if ((k - d.p_off) < d.npb || !d.active) return;
+    const IConst c = make_iconst(d);
+    const float r2cut = d.r2_cut;
+    const float4 ps = st.pospres[k];
+    IPart pi;
+    load_part(st, k, pi);
+    IAcc a = {};
+    int count = 0;
+    const int cix = cell_coord(ps.x, c.i_h), ciy = cell_coord(ps.y, c.i_h), ciz = cell_coord(ps.z, c.i_h);
+    for_each_column(d, cix, ciy, ciz, 0, 1, beg, end, [&](int jb, int je) {
+        for (int j = jb; j < je; ++j) {
+            if (j == k) continue;
+            const float4 pj = st.pospres[j];
+            if (r2_fp32(ps.x - pj.x, ps.y - pj.y, ps.z - pj.z) < r2cut) {
+                ++count;
+                pair_eval(pi, j, st, c, a);
+            }
+        }
+    });
+    store_rates(d, rt, k, a);
+    if (nbr) nbr[k] = count;
+}
+
+// Variant 1: the candidate scan appends hits to a per-lane list in LDS; the
+// wave evaluates the lists when any lane's list is full and at the end, so
+// the ~60-flop pair body runs with most lanes busy instead of once per
+// candidate with ~15% of lanes busy.  Summation order per particle is the
+// candidate order, identical to variant 0.
+constexpr int IC_CAP = 32;
+
+template <class DP>
+__global__ __launch_bounds__(256) void k_interaction_lds(DP dp, DevState st, DevRates rt,
+                                                         const int32_t* __restrict__ beg,
+                                                         const int32_t* __restrict__ end, int n,
+                                                         int32_t* __restrict__ nbr) {
+    __shared__ int32_t lists[4][IC_CAP][64];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int32_t(*my)[64] = lists[w];
+    bool mat = false;
+    const RunDesc* dptr = nullptr;
+    if (k < n) {
+        const RunDesc& d = dp.get(dp.run(k));
+        dptr = &d;
+        mat = (k - d.p_off) >= d.npb && d.active;
+    }
+    IConst c = {};
+    float r2cut = 0.f;
+    float4 ps = make_float4(0.f, 0.f, 0.f, 0.f);
+    IPart pi = {};
+    int cix = 0, ciy = 0, ciz = 0, nx = 0, ny = 0, nz = 0, nol = 0, cbase = 0;
+    if (mat) {
+        const RunDesc& d = *dptr;
+        c = make_iconst(d);
+        r2cut = d.r2_cut;
+        ps = st.pospres[k];
+        load_part(st, k, pi);
+        cix = cell_coord(ps.x, c.i_h);
+        ciy = cell_coord(ps.y, c.i_h);
+        ciz = cell_coord(ps.z, c.i_h);
+        nx = d.par.cell_num_x;
+        ny = d.par.cell_num_y;
+        nz = d.par.cell_num_z;
+        nol = d.par.ncell_one_layer;
+        cbase = d.cell_base;
+    }
+    IAcc a = {};
+    int count = 0, cnt = 0;
+    const int czlo = max(ciz - 1, 0), czhi = min(ciz + 1, nz - 1);
+    for (int col = 0; col < 18; ++col) {
+        const int layer = col / 9, ox = (col % 9) / 3 - 1, oy = col % 3 - 1;
+        int jb = 0, je = 0;
+        if (mat && czlo <= czhi) {
+            const int cx = cix + ox, cy = ciy + oy;
+            if (cx >= 0 && cx < nx && cy >= 0 && cy < ny) {
+                const int c0 = cbase + layer * nol + (cx * ny + cy) * nz;
+                int b0 = -1, e0 = -1;
+                for (int cz = czlo; cz <= czhi; ++cz) {
+                    const int b = beg[c0 + cz];
+                    if (b != -1) {
+                        if (b0 < 0) b0 = b;
+                        e0 = end[c0 + cz];
+                    }
+                }
+                if (b0 >= 0) {
+                    jb = b0;
+                    je = e0;
+                }
+            }
+        }
+        int j = jb;
+        while (__ballot(j < je)) {
+            if (j < je) {
+                if (j != k) {
+                    const float4 pj = st.pospres[j];
+                    if (r2_fp32(ps.x - pj.x, ps.y - pj.y, ps.z - pj.z) < r2cut) {
+                        my[cnt][lane] = j;
+                        ++cnt;
+                    }
+                }
+                ++j;
+            }
+            if (__ballot(cnt == IC_CAP)) {
+                for (int e = 0; __ballot(e < cnt); ++e) {
+                    if (e < cnt) pair_eval(pi, my[e][lane], st, c, a);
+                }
+                count += cnt;
+                cnt = 0;
+            }
+        }
+    }
+    for (int e = 0; __ballot(e < cnt); ++e) {
+        if (e < cnt) pair_eval(pi, my[e][lane], st, c, a);
+    }
+    count += cnt;
+    if (mat) {
+        store_rates(*dptr, rt, k, a);
+        if (nbr) nbr[k] = count;
+    }
+}
+
+// ------------------------------------------------------ constitutive models
+// Drucker-Prager (src/predictor_corrector.cu:30-105): FP32 Jaumann + elastic
+// trial, FP64 return mapping (tensile cut-off, then shear return with
+// non-associated dilatancy).
+__device__ __forceinline__ void mat_dp(const RunDesc& d, float dt, const float4 w, const tensor2d& D, float& p,
+                                       float& ep, tensor2d& s) {
+    const float K = d.par.model_par_1, G = d.par.model_par_2;
+    tensor2d r = jaumann(w, s);
+    r = tadd(r, tscale(D, 2.0f * G));
+    const float ace = (K - 2.0f / 3.0f * G) * (D.xx + D.yy + D.zz);
+    r.xx += ace;
+    r.yy += ace;
+    r.zz += ace;
+    s = tadd(s, tscale(r, dt));
+
+    const double Kd = K, Gd = G, kphi = d.par.model_par_3, kpsi = d.par.model_par_4, kc = d.par.model_par_5;
+    double sm = ((double)s.xx + (double)s.yy + (double)s.zz) / 3.0;
+    double xx = (double)s.xx - sm, yy = (double)s.yy - sm, zz = (double)s.zz - sm;
+    double xy = s.xy, xz = s.xz, yz = s.yz;
+    double epd = ep;
+    const double tenf = (d.par.model_par_3 == 0.0f) ? 0.0 : kc / kphi;
+    const double dpti = sm - tenf;
+    if (dpti >= 0.0) {
+        epd += dpti / Kd * (1.0 / 3.0) * 1.4142135623730951;
+        sm = tenf;
+    }
+    const double tau = sqrt(0.5 * (xx * xx + yy * yy + zz * zz + 2.0 * (xy * xy + xz * xz + yz * yz)));
+    const double dpfi = tau + kphi * sm - kc;
+    if (dpfi > 0.0) {
+        const double dl = dpfi / (Gd + Kd * kphi * kpsi);
+        sm -= Kd * kpsi * dl;
+        const double ratio = tau < 1e-10 ? 1.0 : (kc - kphi * sm) / tau;
+        xx *= ratio;
+        yy *= ratio;
+        zz *= ratio;
+        xy *= ratio;
+        xz *= ratio;
+        yz *= ratio;
+        epd += dl * sqrt(1.0 / 3.0 + 2.0 / 9.0 * kpsi * kpsi);
+    }
+    s = tensor2d{(float)(xx + sm), (float)(yy + sm), (float)(zz + sm), (float)xy, (float)xz, (float)yz};
+    p = (float)(-sm);
+    ep = (float)epd;
+}
+
+// Wu-type 4-constant hypoplastic rate (src/predictor_corrector.cu:111-165),
+// FP32, including the reference's small-trace guard exactly as written there
+// (it resets xy twice and leaves xz, :140-141).
+__device__ __forceinline__ void mat_wu(const RunDesc& d, float dt, const tensor2d& D, const float4 w, tensor2d& s) {
+    const tensor2d jau = jaumann(w, s);
+    const float c1 = d.par.model_par_1, c2 = d.par.model_par_2, c3 = d.par.model_par_3, c4 = d.par.model_par_4,
+                coh = d.par.model_par_5;
+    tensor2d t = s;
+    if (coh > 0.0f) {
+        t.xx -= coh;
+        t.yy -= coh;
+        t.zz -= coh;
+    }
+    float t1 = t.xx + t.yy + t.zz;
+    if (fabsf(t1) < 4.0f) {
+        t1 = -4.0f;
+        t.xx = -1.0f;
+        t.yy = -1.0f;
+        t.zz = -2.0f;
+        t.xy = 0.0f;
+        t.yz = 0.0f;
+    }
+    tensor2d ts = t;
+    ts.xx -= t1 / 3.0f;
+    ts.yy -= t1 / 3.0f;
+    ts.zz -= t1 / 3.0f;
+    const float d1 = D.xx + D.yy + D.zz;
+    const float t1d1 = tdot(t, D);
+    const float nd = sqrtf(tdot(D, D));
+    tensor2d rate = tadd(jau, tscale(D, c1 * t1));
+    rate = tadd(rate, tscale(t, c2 * d1 + c3 * t1d1 / t1));
+    rate = tadd(rate, tscale(tadd(t, ts), c4 * nd));
+    s = tadd(s, tscale(rate, dt));
+    const float pres = (s.xx + s.yy + s.zz) / 3.0f;
+    if (pres > coh) {
+        s.xx -= (pres - coh);
+        s.yy -= (pres - coh);
+        s.zz -= (pres - coh);
+    }
+}
+
+// von Wolffersdorff hypoplasticity (constitutive model 3; von Wolffersdorff
+// 1996, Mech. Cohes.-Frict. Mater. 1:251-271, with the Bauer compression law
+// of Herle & Gudehus 1999).  Tension-positive stress; p = -tr(T)/3 is floored
+// at VW_PMIN for the rate evaluation.  Updates stress and void ratio.
+constexpr float VW_PMIN = 1.0f;
+
+__device__ __forceinline__ void mat_vw(const RunDesc& d, float dt, const tensor2d& D, const float4 w, tensor2d& s,
+                                       float& e) {
+    const tensor2d jau = jaumann(w, s);
+    const float trT = s.xx + s.yy + s.zz;
+    float p;
+    tensor2d th;  // T / tr T
+    if (trT > -3.0f * VW_PMIN) {
+        p = VW_PMIN;
+        th = tensor2d{1.0f / 3.0f, 1.0f / 3.0f, 1.0f / 3.0f, 0.f, 0.f, 0.f};
+    } else {
+        p = -trT / 3.0f;
+        th = tscale(s, 1.0f / trT);
+    }
+    tensor2d ts = th;  // deviator of th
+    ts.xx -= 1.0f / 3.0f;
+    ts.yy -= 1.0f / 3.0f;
+    ts.zz -= 1.0f / 3.0f;
+    const float tt = tdot(th, th);
+    const float ss = tdot(ts, ts);
+    const float tpsi = 1.7320508f * sqrtf(ss);
+    float c3t = 0.0f;
+    if (ss > 1e-12f) {
+        const float det = ts.xx * (ts.yy * ts.zz - ts.yz * ts.yz) - ts.xy * (ts.xy * ts.zz - ts.yz * ts.xz) +
+                          ts.xz * (ts.xy * ts.yz - ts.yy * ts.xz);
+        c3t = -2.4494897f * 3.0f * det / (ss * sqrtf(ss));
+        c3t = fminf(fmaxf(c3t, -1.0f), 1.0f);
+    }
+    const float farg = tpsi * tpsi / 8.0f + (2.0f - tpsi * tpsi) / (2.0f + 1.4142135f * tpsi * c3t);
+    const float F = sqrtf(fmaxf(farg, 0.0f)) - tpsi / (2.0f * 1.4142135f);
+    const float a = d.vw_a;
+    const float hs = d.vw.hs, nn = d.vw.n;
+    const float x = 3.0f * p / hs;
+    const float ex = expf(-powf(x, nn));
+    const float ei = d.vw.ei0 * ex, ec = d.vw.ec0 * ex, ed = d.vw.ed0 * ex;
+    const float fe = powf(ec / e, d.vw.beta);
+    const float fd = powf(fmaxf((e - ed) / (ec - ed), 0.0f), d.vw.alpha);
+    const float fb = d.vw_fb0 * (1.0f + ei) / ei * powf(x, 1.0f - nn);
+    const float fs = fb * fe;
+    const float nd = sqrtf(tdot(D, D));
+    const float thd = tdot(th, D);
+    tensor2d rate = tscale(D, F * F);
+    rate = tadd(rate, tscale(th, a * a * thd));
+    rate = tadd(rate, tscale(tadd(th, ts), fd * a * F * nd));
+    rate = tscale(rate, fs / tt);
+    s = tadd(s, tscale(tadd(jau, rate), dt));
+    e = e + dt * (1.0f + e) * (D.xx + D.yy + D.zz);
+}
+
+__device__ __forceinline__ void constitutive(const RunDesc& d, float dt, const float4 w, const tensor2d& D,
+                                             const tensor2d& stn_new, float& p, float& ep, float& voidr,
+                                             tensor2d& s) {
+    const int model = d.par.constitutive_model;
+    if (model == 1) {
+        mat_dp(d, dt, w, D, p, ep, s);
+    } else if (model == 2) {
+        mat_wu(d, dt, D, w, s);
+        ep = sqrtf(2.0f / 3.0f * tdot(stn_new, stn_new));
+    } else {
+        mat_vw(d, dt, D, w, s, voidr);
+        ep = sqrtf(2.0f / 3.0f * tdot(stn_new, stn_new));
+        p = -(s.xx + s.yy + s.zz) / 3.0f;
+    }
+}
+
+// Predictor (src/predictor_corrector.cu:169-218) fused with the *_ini copy
+// that the reference does as five device-to-device copies (:240-246).
+template <class DP>
+__global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni ini, DevRates rt, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const RunDesc& d = dp.get(dp.run(k));
+    if ((k - d.p_off) < d.npb || !d.active) return;
+    const float dt = d.par.dt;
+    const float hdt = 0.5f * dt;
+    const double hdtd = (double)hdt;
+    const float4 a = rt.accad[k];
+    float4 vr = st.velrhop[k];
+    float4 idep = st.idep[k];
+    const tensor2d sig0 = st.str[k];
+    const tensor2d stn0 = st.stn[k];
+    double x = st.xd[k], y = st.yd[k], z = st.zd[k];
+    ini.xd[k] = x;
+    ini.yd[k] = y;
+    ini.zd[k] = z;
+    ini.velrhop[k] = vr;
+    ini.epvr[k] = make_float2(idep.y, idep.z);
+    ini.str[k] = sig0;
+    ini.stn[k] = stn0;
+
+    vr.x += hdt * a.x;
+    vr.y += hdt * a.y;
+    vr.z += hdt * a.z;
+    vr.w += hdt * a.w;
+    x += hdtd * (double)vr.x;
+    y += hdtd * (double)vr.y;
+    z += hdtd * (double)vr.z;
+    const tensor2d D = rt.acestn[k];
+    const tensor2d stn = tadd(stn0, tscale(D, hdt));
+    float p = st.pospres[k].w;
+    float ep = idep.y, voidr = idep.z;
+    tensor2d s = sig0;
+    constitutive(d, hdt, rt.spin[k], D, stn, p, ep, voidr, s);
+
+    st.xd[k] = x;
+    st.yd[k] = y;
+    st.zd[k] = z;
+    st.pospres[k] = make_float4((float)x, (float)y, (float)z, p);
+    st.velrhop[k] = vr;
+    idep.y = ep;
+    idep.z = voidr;
+    st.idep[k] = idep;
+    st.str[k] = s;
+    st.stn[k] = stn;
+}
+
+// Corrector (src/predictor_corrector.cu:256-307): midpoint update from ini.
+template <class DP>
+__global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni ini, DevRates rt, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const RunDesc& d = dp.get(dp.run(k));
+    if ((k - d.p_off) < d.npb || !d.active) return;
+    const float dt = d.par.dt;
+    const double dtd = (double)dt;
+    const float4 a = rt.accad[k];
+    const float4 v0 = ini.velrhop[k];
+    float4 vr;
+    vr.x = v0.x + dt * a.x;
+    vr.y = v0.y + dt * a.y;
+    vr.z = v0.z + dt * a.z;
+    vr.w = v0.w + dt * a.w;
+    const double x = 0.5 * ((double)vr.x + (double)v0.x) * dtd + ini.xd[k];
+    const double y = 0.5 * ((double)vr.y + (double)v0.y) * dtd + ini.yd[k];
+    const double z = 0.5 * ((double)vr.z + (double)v0.z) * dtd + ini.zd[k];
+    const tensor2d D = rt.acestn[k];
+    const tensor2d stn = tadd(ini.stn[k], tscale(D, dt));
+    float p = st.pospres[k].w;
+    const float2 epvr = ini.epvr[k];
+    float ep = epvr.x, voidr = epvr.y;
+    tensor2d s = ini.str[k];
+    constitutive(d, dt, rt.spin[k], D, stn, p, ep, voidr, s);
+
+    st.xd[k] = x;
+    st.yd[k] = y;
+    st.zd[k] = z;
+    st.pospres[k] = make_float4((float)x, (float)y, (float)z, p);
+    st.velrhop[k] = vr;
+    float4 idep = st.idep[k];
+    idep.y = ep;
+    idep.z = voidr;
+    st.idep[k] = idep;
+    st.str[k] = s;
+    st.stn[k] = stn;
+}
+
+// ----------------------------------------------------------------- Shepard
+// src/field_smoothing.cu:65-121: material neighbours only, self included.
+// A particle with no neighbour at all (only possible outside the grid) keeps
+// its stress instead of dividing by zero.
+template <class DP>
+__global__ __launch_bounds__(256) void k_shepard(DP dp, DevState st, const tensor2d* __restrict__ src,
+                                                 tensor2d* __restrict__ dst, const int32_t* __restrict__ beg,
+                                                 const int32_t* __restrict__ end, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const RunDesc& d = dp.get(dp.run(k));
+    if ((k - d.p_off) < d.npb || !d.active || !d.shepard_due) {
+        dst[k] = src[k];
+        return;
+    }
+    const float4 pi = st.pospres[k];
+    const double xi = st.xd[k], yi = st.yd[k], zi = st.zd[k];
+    const float i_h = d.par.i_h, cw = d.par.kernel_normalization_par, m = d.par.m, r2cut = d.r2_cut;
+    float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f, w = 0.f;
+    const int cix = cell_coord(pi.x, i_h), ciy = cell_coord(pi.y, i_h), ciz = cell_coord(pi.z, i_h);
+    for_each_column(d, cix, ciy, ciz, 1, 1, beg, end, [&](int jb, int je) {
+        for (int j = jb; j < je; ++j) {
+            const float4 pj = st.pospres[j];
+            if (r2_fp32(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < r2cut) {
+                const float dx = (float)(xi - st.xd[j]), dy = (float)(yi - st.yd[j]), dz = (float)(zi - st.zd[j]);
+                const float q = fsqrt(dx * dx + dy * dy + dz * dz) * i_h;
+                const float t = fmaxf(2.0f - q, 0.0f);
+                const float t2 = t * t;
+                const float kv = cw * t2 * t2 * (q + 0.5f) * m * frcp(st.velrhop[j].w);
+                const tensor2d s = src[j];
+                sxx += s.xx * kv;
+                syy += s.yy * kv;
+                szz += s.zz * kv;
+                sxy += s.xy * kv;
+                sxz += s.xz * kv;
+                syz += s.yz * kv;
+                w += kv;
+            }
+        }
+    });
+    if (w > 0.0f) {
+        const float iw = 1.0f / w;
+        dst[k] = tensor2d{sxx * iw, syy * iw, szz * iw, sxy * iw, sxz * iw, syz * iw};
+    } else {
+        dst[k] = src[k];
+    }
+}
+
+__global__ void k_step_begin(RunDesc* d, int nruns) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nruns) return;
+    const int f = d[r].par.stress_shepard_filter;
+    const int step = d[r].step;
+    d[r].shepard_due = (d[r].active && f != 0 && step % f == 0 && step != 0) ? 1 : 0;
+}
+
+__global__ void k_step_end(RunDesc* d, int nruns) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nruns) return;
+    if (!d[r].active) return;
+    d[r].time += (double)d[r].par.dt;
+    d[r].step += 1;
+    if (d[r].step >= d[r].nsteps) d[r].active = 0;
+}
+
+__global__ void k_copy_tensor(const tensor2d* __restrict__ src, tensor2d* __restrict__ dst, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[k];
+}
+
+}  // namespace
+
+void launch_adami_one(const RunDesc& d, const DevState& st, const int32_t* beg, const int32_t* end, int n,
+                      hipStream_t s) {
+    if (n > 0) k_adami<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, beg, end, n);
+}
+void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const int32_t* beg,
+                        const int32_t* end, int n, hipStream_t s) {
+    if (n > 0) k_adami<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, st, beg, end, n);
+}
+
+template <class DP>
+static void launch_interaction(const DP& dp, const DevState& st, const DevRates& rt, const int32_t* beg,
+                               const int32_t* end, int n, int32_t* nbr, int variant, hipStream_t s) {
+    if (n <= 0) return;
+    if (variant == 0)
+        k_interaction_direct<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, rt, beg, end, n, nbr);
+    else
+        k_interaction_lds<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, rt, beg, end, n, nbr);
+}
+
+void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates& rt, const int32_t* beg,
+                            const int32_t* end, int n, int32_t* nbr, int variant, hipStream_t s) {
+    launch_interaction(DescOne{d}, st, rt, beg, end, n, nbr, variant, s);
+}
+void launch_interaction_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt,
+                              const int32_t* beg, const int32_t* end, int n, int32_t* nbr, int variant,
+                              hipStream_t s) {
+    launch_interaction(DescTable{d, run_of}, st, rt, beg, end, n, nbr, variant, s);
+}
+
+void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, const DevIni& ini,
+                          const DevRates& rt, int n, hipStream_t s) {
+    if (n <= 0) return;
+    if (corrector)
+        k_corrector<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, n);
+    else
+        k_predictor<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, n);
+}
+void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
+                            const DevIni& ini, const DevRates& rt, int n, hipStream_t s) {
+    if (n <= 0) return;
+    if (corrector)
+        k_corrector<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, n);
+    else
+        k_predictor<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, n);
+}
+
+void launch_shepard_one(const RunDesc& d, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,
+                        const int32_t* beg, const int32_t* end, int n, hipStream_t s) {
+    if (n > 0) k_shepard<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, str_src, str_dst, beg, end, n);
+}
+void launch_shepard_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const tensor2d* str_src,
+                          tensor2d* str_dst, const int32_t* beg, const int32_t* end, int n, hipStream_t s) {
+    if (n > 0)
+        k_shepard<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, st, str_src, str_dst, beg, end,
+                                                                 n);
+}
+
+void launch_step_begin(RunDesc* d, int nruns, hipStream_t s) {
+    if (nruns > 0) k_step_begin<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns);
+}
+void launch_step_end(RunDesc* d, int nruns, hipStream_t s) {
+    if (nruns > 0) k_step_end<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns);
+}
+void launch_copy_tensor(const tensor2d* src, tensor2d* dst, int n, hipStream_t s) {
+    if (n > 0) k_copy_tensor<<<blocks_for(n, 256), 256, 0, s>>>(src, dst, n);
+}
+
+}  // namespace lq

@@ -1,0 +1,104 @@
+"""Engine (time loop) on the GPU: trajectory against the FP64 oracle, and
+batching invariants -- a run's result is bit-identical alone, in a batch,
+with graphs or direct launches, and across joins/leaves of other runs."""
+import numpy as np
+import pytest
+
+from helpers import rel_err
+from loquat_amd import cases
+from oracle import lq_oracle as O
+
+
+def _engine(cases_, steps=None, graphs=True):
+    from loquat_amd.device import Engine
+    np_cap = sum(c.np for c in cases_)
+    nc = sum(c.derived().ncell for c in cases_)
+    e = Engine(np_cap, nc, max_runs=max(1, len(cases_)))
+    e.set_graphs(graphs)
+    ids = [e.add(c, steps) for c in cases_]
+    return e, ids
+
+
+def _same(a: dict, b: dict) -> bool:
+    return all(np.array_equal(a[k], b[k], equal_nan=True) for k in ("posd", "pospres", "velrhop", "idep", "str", "stn"))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model", [1, 2, 3])
+def test_engine_vs_oracle_steps(gpu, model):
+    """20 full steps (with Shepard every 5) from the same start."""
+    from test_gpu_parity import model_case
+    case = model_case(model, seed=21)
+    case.par.stress_shepard_filter = 5
+    nsteps = 20
+    e, (rid,) = _engine([case], nsteps)
+    e.step(nsteps)
+    got = e.download(rid)
+    assert e.info(rid)["steps"] == nsteps and e.info(rid)["finished"]
+    p = case.derived()
+    st = O.OState.from_case(case)
+    O.initial_sort(p, st)
+    O.run(p, case.vw, st, nsteps)
+    ref = O.unpermute(st)
+    npb = p.npb
+    m = slice(npb, None)
+    x0 = case.posd
+    for a, k in enumerate(("x", "y", "z")):
+        assert rel_err(got["posd"][m, a] - x0[m, a], getattr(ref, k)[m] - x0[m, a]) < 1e-4, k
+    for a, k in enumerate(("vx", "vy", "vz")):
+        assert rel_err(got["velrhop"][m, a], getattr(ref, k)[m]) < 1e-4, k
+    assert rel_err(got["velrhop"][m, 3] - p.rho0, ref.rho[m] - p.rho0) < 1e-3
+    for c in range(6):
+        assert rel_err(got["str"][m, c], ref.s[c, m]) < 1e-4, c
+    e.close()
+
+
+@pytest.mark.gpu
+def test_batch_equals_alone(gpu):
+    runs = cases.ensemble(nruns=5, seed=3, nsteps=30, npm_lo=500, npm_hi=4000)
+    alone = []
+    for c in runs:
+        e, (rid,) = _engine([c])
+        e.step(30)
+        alone.append(e.download(rid))
+        e.close()
+    e, ids = _engine(runs)
+    e.step(12)
+    # a run leaves and another joins mid-way; the others must not notice
+    e.remove(ids[1])
+    extra = cases.ensemble(nruns=1, seed=99, nsteps=30, npm_lo=500, npm_hi=4000)[0]
+    e.add(extra)
+    e.step(18)
+    for i, rid in enumerate(ids):
+        if i == 1:
+            continue
+        assert _same(e.download(rid), alone[i]), i
+    e.close()
+
+
+@pytest.mark.gpu
+def test_graphs_equal_direct(gpu):
+    c = cases.perturb(cases.small_column_3d(seed=31, shepard=3), seed=32)
+    out = []
+    for graphs in (True, False):
+        e, (rid,) = _engine([c], 10, graphs=graphs)
+        e.step(10)
+        out.append(e.download(rid))
+        e.close()
+    assert _same(out[0], out[1])
+
+
+@pytest.mark.gpu
+def test_interaction_variants_agree_in_engine(gpu):
+    from loquat_amd import abi
+    c = cases.perturb(cases.small_column_3d(seed=41), seed=42)
+    res = []
+    for v in (0, 1):
+        abi.check(abi.lib().lq_set_interaction_variant(v))
+        e, (rid,) = _engine([c], 5)
+        e.step(5)
+        res.append(e.download(rid))
+        e.close()
+    abi.check(abi.lib().lq_set_interaction_variant(1))
+    for k in ("posd", "velrhop", "str"):
+        assert rel_err(res[0][k], res[1][k]) < 1e-5

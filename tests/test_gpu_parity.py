@@ -1,0 +1,252 @@
+"""Per-call parity of every HIP stage against the FP64 oracle on identical
+inputs, through the C ABI.
+
+Bars: grid (cell ids, sort order, cell table) and neighbour counts bit-exact;
+fields within 1e-5 of the oracle, normwise per field:
+max_i |gpu - oracle| / max_i |oracle|.
+"""
+import numpy as np
+import pytest
+
+from helpers import rel_err
+from loquat_amd import abi, cases
+from oracle import lq_oracle as O
+
+TOL = 1e-5
+WU = (-101.2, -962.1, -877.3, 1229.0, 0.0)
+
+
+def _dev():
+    from loquat_amd import device
+    return device
+
+
+def rounded_state(st: O.OState):
+    """FP32 roundings of a state: the host arrays the GPU gets, and the
+    oracle state holding exactly the same values (FP64 positions kept)."""
+    n = st.n
+    pospres = np.zeros((n, 4), np.float32)
+    pospres[:, 0], pospres[:, 1], pospres[:, 2] = st.x, st.y, st.z
+    pospres[:, 3] = st.p
+    velrhop = np.stack([st.vx, st.vy, st.vz, st.rho], 1).astype(np.float32)
+    idep = np.zeros((n, 4), np.float32)
+    idep[:, 0], idep[:, 1], idep[:, 2] = st.id, st.ep, st.voidr
+    s = st.s.T.astype(np.float32)
+    e = st.e.T.astype(np.float32)
+    posd = st.posd.copy()
+    host = dict(posd=posd, pospres=pospres, velrhop=velrhop, idep=idep, str=s, stn=e, orig=st.orig.copy())
+    ost = O.OState.from_arrays(posd, pospres, velrhop, idep, s, e, st.orig)
+    return host, ost
+
+
+def sorted_inputs(case):
+    p = case.derived()
+    st = O.OState.from_case(case)
+    O.initial_sort(p, st)
+    host, ost = rounded_state(st)
+    ds = _dev().DeviceState.from_host(host["posd"], host["pospres"], host["velrhop"], host["idep"], host["str"],
+                                      host["stn"], host["orig"])
+    return p, ds, ost
+
+
+def model_case(model: int, seed: int = 3):
+    c = cases.perturb(cases.small_column_3d(nx=14, ny=8, nz=20, seed=seed), seed=seed + 1)
+    if model == 2:
+        c.par.constitutive_model = 2
+        (c.par.model_par_1, c.par.model_par_2, c.par.model_par_3, c.par.model_par_4, c.par.model_par_5) = WU
+    elif model == 3:
+        c.par.constitutive_model = 3
+        c.vw = abi.LqVw(**cases.VW_SAND)
+        rng = np.random.default_rng(seed)
+        c.idep[c.par.npb:, 2] = (0.7 * (1 + rng.uniform(-0.05, 0.05, c.npm))).astype(np.float32)
+    return c
+
+
+def cases_for_grid():
+    c1 = cases.column_2d_small(nsteps=1)
+    s3 = cases.perturb(cases.small_column_3d(seed=5), seed=6)
+    # shuffled order inside each block, some material particles outside the grid
+    sh = cases.perturb(cases.small_column_3d(seed=7), seed=8)
+    rng = np.random.default_rng(9)
+    npb, n = sh.par.npb, sh.par.np
+    order = np.concatenate([rng.permutation(npb), npb + rng.permutation(n - npb)])
+    for k in ("pospres", "velrhop", "idep", "str", "stn", "posd"):
+        setattr(sh, k, np.ascontiguousarray(getattr(sh, k)[order]))
+    out = npb + rng.choice(n - npb, 25, replace=False)
+    sh.posd[out[:10], 0] = -0.05
+    sh.posd[out[10:20], 2] = 10.0
+    sh.posd[out[20:], 1] = np.nan
+    sh.pospres[:, :3] = sh.posd.astype(np.float32)
+    return {"c1": c1, "small3d": s3, "shuffled_outside": sh}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["c1", "small3d", "shuffled_outside"])
+def test_grid_bit_exact(gpu, name):
+    dev = _dev()
+    case = cases_for_grid()[name]
+    p = case.derived()
+    st = O.OState.from_case(case)
+    ref = O.grid_build(p, st)
+    ds = dev.DeviceState.from_case(case)
+    g = dev.Grid(case.np, p.ncell)
+    g.build(p, ds)
+    got = g.arrays(case.np, p.ncell)
+    for k in ("part_cel", "part_idx", "cell_beg", "cell_end"):
+        assert np.array_equal(got[k], ref[k]), k
+    # the permute follows part_idx exactly
+    dst = dev.DeviceState(case.np)
+    g.permute(case.np, ds, dst)
+    h = dst.host()
+    perm = ref["part_idx"]
+    assert np.array_equal(h["orig"], perm)
+    assert np.array_equal(h["posd"], case.posd[perm], equal_nan=True)
+    assert np.array_equal(h["str"], case.str[perm])
+    g.close()
+
+
+@pytest.mark.gpu
+def test_grid_full_size_c2(gpu):
+    """Grid of the 1.55 M-particle 3D column: bit-exact against the oracle."""
+    dev = _dev()
+    case = cases.column_3d(nsteps=1)
+    p = case.derived()
+    ref = O.grid_build(p, O.OState.from_case(case))
+    ds = dev.DeviceState.from_case(case)
+    g = dev.Grid(case.np, p.ncell)
+    g.build(p, ds)
+    got = g.arrays(case.np, p.ncell)
+    for k in ("part_cel", "part_idx", "cell_beg", "cell_end"):
+        assert np.array_equal(got[k], ref[k]), k
+    assert np.all(np.diff(got["part_cel"]) >= 0)
+    g.close()
+
+
+def _grid_tables(p, ds, ost):
+    dev = _dev()
+    g = dev.Grid(ds.n, p.ncell)
+    g.build(p, ds)
+    beg, end = g.table_ptrs()
+    ref = O.grid_build(p, ost)
+    assert np.array_equal(ref["part_idx"], np.arange(ds.n)), "input must already be sorted"
+    return g, beg, end, ref
+
+
+@pytest.mark.gpu
+def test_adami_parity(gpu):
+    dev = _dev()
+    p, ds, ost = sorted_inputs(cases.perturb(cases.small_column_3d(seed=11), seed=12))
+    g, beg, end, ref = _grid_tables(p, ds, ost)
+    dev.adami(p, ds, beg, end)
+    O.adami(p, ost, ref["cell_beg"], ref["cell_end"])
+    h = ds.host()
+    npb = p.npb
+    for c in range(6):
+        assert rel_err(h["str"][:npb, c], ost.s[c, :npb]) < TOL, c
+    g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("which", ["small3d", "c1"])
+def test_interaction_parity(gpu, variant, which):
+    dev = _dev()
+    base = cases.small_column_3d(seed=13) if which == "small3d" else cases.column_2d_small(nsteps=1)
+    p, ds, ost = sorted_inputs(cases.perturb(base, seed=14))
+    g, beg, end, ref = _grid_tables(p, ds, ost)
+    abi.check(abi.lib().lq_set_interaction_variant(variant))
+    try:
+        import torch
+        rt = dev.DeviceRates(ds.n)
+        nbr = torch.full((ds.n,), -1, dtype=torch.int32, device="cuda")
+        dev.interaction(p, ds, rt, beg, end, nbr)
+        ort, onbr = O.interaction(p, ost, ref["cell_beg"], ref["cell_end"])
+        h = rt.host()
+        nb = nbr.cpu().numpy()
+    finally:
+        abi.check(abi.lib().lq_set_interaction_variant(1))
+    npb = p.npb
+    assert np.array_equal(nb[npb:], onbr[npb:])
+    assert nb[npb:].min() > 0
+    m = slice(npb, None)
+    assert rel_err(h["accad"][m, 0], ort.ax[m]) < TOL
+    assert rel_err(h["accad"][m, 1], ort.ay[m]) < TOL
+    assert rel_err(h["accad"][m, 2], ort.az[m]) < TOL
+    assert rel_err(h["accad"][m, 3], ort.drho[m]) < TOL
+    assert rel_err(h["spin"][m, 0], ort.wx[m]) < TOL
+    assert rel_err(h["spin"][m, 1], ort.wy[m]) < TOL
+    assert rel_err(h["spin"][m, 2], ort.wz[m]) < TOL
+    for c in range(6):
+        assert rel_err(h["acestn"][m, c], ort.d[c, m]) < TOL, c
+    g.close()
+
+
+def _rates_from_oracle(ort: O.ORates, n: int):
+    dev = _dev()
+    import torch
+    rt = dev.DeviceRates(n)
+    acc = np.stack([ort.ax, ort.ay, ort.az, ort.drho], 1).astype(np.float32)
+    spin = np.stack([ort.wx, ort.wy, ort.wz, np.zeros(n)], 1).astype(np.float32)
+    rt.accad.copy_(torch.from_numpy(acc))
+    rt.spin.copy_(torch.from_numpy(spin))
+    rt.acestn.copy_(torch.from_numpy(ort.d.T.astype(np.float32).copy()))
+    return rt, O.ORates.from_device(rt.host())
+
+
+def _check_state(p, h, ost, before, tol=TOL):
+    npb = p.npb
+    m = slice(npb, None)
+    for a, k in enumerate(("x", "y", "z")):
+        # compare displacements, not absolute positions
+        d_gpu = h["posd"][m, a] - before.posd[m, a]
+        d_ref = getattr(ost, k)[m] - before.posd[m, a]
+        assert rel_err(d_gpu, d_ref) < tol, k
+    for a, k in enumerate(("vx", "vy", "vz", "rho")):
+        assert rel_err(h["velrhop"][m, a], getattr(ost, k)[m]) < tol, k
+    for c in range(6):
+        assert rel_err(h["str"][m, c], ost.s[c, m]) < tol, ("str", c)
+        assert rel_err(h["stn"][m, c], ost.e[c, m]) < tol, ("stn", c)
+    assert rel_err(h["idep"][m, 1], ost.ep[m], floor=1e-12) < tol
+    assert rel_err(h["pospres"][m, 3], ost.p[m]) < tol
+    assert rel_err(h["idep"][m, 2], ost.voidr[m], floor=1e-12) < tol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model", [1, 2, 3])
+def test_predictor_corrector_parity(gpu, model):
+    dev = _dev()
+    case = model_case(model)
+    p, ds, ost = sorted_inputs(case)
+    g, beg, end, ref = _grid_tables(p, ds, ost)
+    ort, _ = O.interaction(p, ost, ref["cell_beg"], ref["cell_end"])
+    rt, ort = _rates_from_oracle(ort, ds.n)
+    before = ost.copy()
+    ini = dev.DeviceIni(ds.n)
+    oini = O.OIni(ds.n)
+    dev.predictor(p, case.vw, ds, ini, rt)
+    O.predictor(p, case.vw, ost, oini, ort)
+    h = ds.host()
+    _check_state(p, h, ost, before)
+    # corrector from the predicted state and the same rates
+    dev.corrector(p, case.vw, ds, ini, rt)
+    O.corrector(p, case.vw, ost, oini, ort)
+    _check_state(p, ds.host(), ost, before)
+    g.close()
+
+
+@pytest.mark.gpu
+def test_shepard_parity(gpu):
+    dev = _dev()
+    import torch
+    p, ds, ost = sorted_inputs(cases.perturb(cases.small_column_3d(seed=17), seed=18))
+    g, beg, end, ref = _grid_tables(p, ds, ost)
+    cache = torch.zeros(ds.n, 6, dtype=torch.float32, device="cuda")
+    walls_before = ds.host()["str"][: p.npb].copy()
+    dev.smoothing(p, ds, cache, beg, end)
+    O.shepard(p, ost, ref["cell_beg"], ref["cell_end"])
+    h = ds.host()
+    m = slice(p.npb, None)
+    for c in range(6):
+        assert rel_err(h["str"][m, c], ost.s[c, m]) < TOL, c
+    assert np.array_equal(h["str"][: p.npb], walls_before)
+    g.close()
