@@ -624,6 +624,22 @@ void lqo_run(const lq_params* par, const lq_vw_params* vw, lqo_state* st, int ns
     }
 }
 
+// One constitutive update at a single material point (the constitutive
+// branch of Predictor_cuk / Corrector_cuk): stress s[6], ep, voidr, p in/out.
+void lqo_constitutive(const lq_params* par, const lq_vw_params* vw, double dt, const double* spin3,
+                      const double* d6, const double* stn6, double* p, double* ep, double* voidr, double* s6) {
+    T6 D{d6[0], d6[1], d6[2], d6[3], d6[4], d6[5]};
+    T6 stn{stn6[0], stn6[1], stn6[2], stn6[3], stn6[4], stn6[5]};
+    T6 s{s6[0], s6[1], s6[2], s6[3], s6[4], s6[5]};
+    constitutive(par, vw, dt, spin3[0], spin3[1], spin3[2], D, stn, *p, *ep, *voidr, s);
+    s6[0] = s.xx;
+    s6[1] = s.yy;
+    s6[2] = s.zz;
+    s6[3] = s.xy;
+    s6[4] = s.xz;
+    s6[5] = s.yz;
+}
+
 int lqo_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -55,6 +55,7 @@ def lib() -> C.CDLL:
         L.lqo_initial_sort.argtypes = [vp, C.POINTER(_State)]
         L.lqo_run.argtypes = [vp, vp, C.POINTER(_State), C.c_int, C.c_int]
         L.lqo_num_threads.restype = C.c_int
+        L.lqo_constitutive.argtypes = [vp, vp, C.c_double, dp, dp, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -180,6 +181,8 @@ class ORates:
 
 
 def _p(par):
+    if par is None:
+        return None
     return C.cast(C.pointer(par), C.c_void_p)
 
 
@@ -268,3 +271,17 @@ def unpermute(st: OState) -> OState:
     o.e[:, idx] = st.e
     o.orig[:] = np.arange(st.n, dtype=np.int32)
     return o
+
+
+def constitutive(par, vw, dt: float, spin3, d6, s6, stn6=None, p: float = 0.0, ep: float = 0.0,
+                 voidr: float = 0.0):
+    """Single-point constitutive update; returns (stress[6], p, ep, voidr)."""
+    w = np.ascontiguousarray(spin3, np.float64)
+    d = np.ascontiguousarray(d6, np.float64)
+    n = np.ascontiguousarray(np.zeros(6) if stn6 is None else stn6, np.float64)
+    s = np.array(s6, np.float64)
+    pp, ee, vv = C.c_double(p), C.c_double(ep), C.c_double(voidr)
+    vwp = _p(vw) if vw is not None else None
+    lib().lqo_constitutive(_p(par), vwp, float(dt), _d(w), _d(d), _d(n), C.byref(pp), C.byref(ee), C.byref(vv),
+                           _d(s))
+    return s, pp.value, ee.value, vv.value

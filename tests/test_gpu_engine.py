@@ -102,3 +102,29 @@ def test_interaction_variants_agree_in_engine(gpu):
     abi.check(abi.lib().lq_set_interaction_variant(1))
     for k in ("posd", "velrhop", "str"):
         assert rel_err(res[0][k], res[1][k]) < 1e-5
+
+
+@pytest.mark.gpu
+def test_c1_full_run_matches_oracle_trajectory(gpu):
+    """Quasi-2D sand column, 20,000 steps on the GPU: runout and free-surface
+    profile within 1% of the FP64 oracle's full run (tests/golden/c1_trajectory.json,
+    written by tests/golden/make_c1_trajectory.py)."""
+    import json
+    import os
+
+    from golden.metrics import collapse_metrics
+
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "c1_trajectory.json")))
+    case = cases.column_2d_small(nsteps=gold["steps"])
+    e, (rid,) = _engine([case])
+    e.step(gold["steps"])
+    got = collapse_metrics(case, e.download(rid)["posd"])
+    e.close()
+    assert abs(got["runout"] - gold["runout"]) <= 0.01 * gold["runout"]
+    assert abs(got["height"] - gold["height"]) <= 0.01 * gold["height"]
+    # free surface: bins present in both, within 1% of the initial column height
+    h0 = case.meta["col"][2]
+    n = min(len(got["profile"]), len(gold["profile"]))
+    diffs = [abs(a - b) for a, b in zip(got["profile"][:n], gold["profile"][:n]) if a is not None and b is not None]
+    assert len(diffs) > 0.8 * n
+    assert max(diffs) <= 0.01 * h0, max(diffs)
