@@ -145,8 +145,8 @@ extern "C" lq_status lq_neighbour_r2_cut(const lq_params* par, float* r2_cut) {
 }
 
 extern "C" lq_status lq_set_interaction_variant(int32_t v) {
-    if (v != 0 && v != 1) {
-        set_error("interaction variant must be 0 or 1");
+    if (v < 0 || v > 2) {
+        set_error("interaction variant must be 0, 1 or 2");
         return LQ_ERR_INVALID_ARG;
     }
     g_variant = v;

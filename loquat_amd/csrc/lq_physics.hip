@@ -329,6 +329,118 @@ __global__ __launch_bounds__(256) void k_interaction_lds(DP dp, DevState st, Dev
     }
 }
 
+// Variant 2: as variant 1, restructured for latency and issue rate.  The
+// candidate loop tests U = 4 candidates per iteration (four loads in flight
+// per lane), appends branch-free (the slot is always written, the count only
+// advances on a hit), and never touches the accumulators: lists are evaluated
+// between column segments, so the loop carries no accumulator registers.
+constexpr int IC2_CAP = 32;
+constexpr int IC2_U = 4;
+
+template <class DP>
+__global__ __launch_bounds__(256) void k_interaction_unrolled(DP dp, DevState st, DevRates rt,
+                                                              const int32_t* __restrict__ beg,
+                                                              const int32_t* __restrict__ end, int n,
+                                                              int32_t* __restrict__ nbr) {
+    __shared__ int32_t lists[4][IC2_CAP][64];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int32_t(*my)[64] = lists[w];
+    bool mat = false;
+    const RunDesc* dptr = nullptr;
+    if (k < n) {
+        const RunDesc& d = dp.get(dp.run(k));
+        dptr = &d;
+        mat = (k - d.p_off) >= d.npb && d.active;
+    }
+    IConst c = {};
+    float r2cut = 0.f;
+    float4 ps = make_float4(0.f, 0.f, 0.f, 0.f);
+    IPart pi = {};
+    int cix = 0, ciy = 0, ciz = 0, nx = 0, ny = 0, nz = 0, nol = 0, cbase = 0;
+    if (mat) {
+        const RunDesc& d = *dptr;
+        c = make_iconst(d);
+        r2cut = d.r2_cut;
+        ps = st.pospres[k];
+        load_part(st, k, pi);
+        cix = cell_coord(ps.x, c.i_h);
+        ciy = cell_coord(ps.y, c.i_h);
+        ciz = cell_coord(ps.z, c.i_h);
+        nx = d.par.cell_num_x;
+        ny = d.par.cell_num_y;
+        nz = d.par.cell_num_z;
+        nol = d.par.ncell_one_layer;
+        cbase = d.cell_base;
+    }
+    IAcc a = {};
+    int count = 0, cnt = 0;
+    const int czlo = max(ciz - 1, 0), czhi = min(ciz + 1, nz - 1);
+    const float4* __restrict__ pp = st.pospres;
+    for (int col = 0; col < 18; ++col) {
+        const int layer = col / 9, ox = (col % 9) / 3 - 1, oy = col % 3 - 1;
+        int j = 0, je = 0;
+        if (mat && czlo <= czhi) {
+            const int cx = cix + ox, cy = ciy + oy;
+            if (cx >= 0 && cx < nx && cy >= 0 && cy < ny) {
+                const int c0 = cbase + layer * nol + (cx * ny + cy) * nz;
+                int b0 = -1, e0 = -1;
+                for (int cz = czlo; cz <= czhi; ++cz) {
+                    const int b = beg[c0 + cz];
+                    if (b != -1) {
+                        if (b0 < 0) b0 = b;
+                        e0 = end[c0 + cz];
+                    }
+                }
+                if (b0 >= 0) {
+                    j = b0;
+                    je = e0;
+                }
+            }
+        }
+        while (true) {
+            while (__ballot(j < je) && !__ballot(cnt > IC2_CAP - IC2_U)) {
+                float4 q[IC2_U];
+#pragma unroll
+                for (int u = 0; u < IC2_U; ++u) {
+                    const int jj = min(j + u, je - 1);
+                    q[u] = pp[max(jj, 0)];
+                }
+#pragma unroll
+                for (int u = 0; u < IC2_U; ++u) {
+                    const int jj = j + u;
+                    const bool hit = jj < je && jj != k &&
+                                     r2_fp32(ps.x - q[u].x, ps.y - q[u].y, ps.z - q[u].z) < r2cut;
+                    my[cnt][lane] = jj;
+                    cnt += hit ? 1 : 0;
+                }
+                j += IC2_U;
+            }
+            if (!__ballot(j < je)) break;
+            for (int e = 0; __ballot(e < cnt); ++e) {
+                if (e < cnt) pair_eval(pi, my[e][lane], st, c, a);
+            }
+            count += cnt;
+            cnt = 0;
+        }
+        if (__ballot(cnt > IC2_CAP - 3 * IC2_U)) {
+            for (int e = 0; __ballot(e < cnt); ++e) {
+                if (e < cnt) pair_eval(pi, my[e][lane], st, c, a);
+            }
+            count += cnt;
+            cnt = 0;
+        }
+    }
+    for (int e = 0; __ballot(e < cnt); ++e) {
+        if (e < cnt) pair_eval(pi, my[e][lane], st, c, a);
+    }
+    count += cnt;
+    if (mat) {
+        store_rates(*dptr, rt, k, a);
+        if (nbr) nbr[k] = count;
+    }
+}
+
 // ------------------------------------------------------ constitutive models
 // Drucker-Prager (src/predictor_corrector.cu:30-105): FP32 Jaumann + elastic
 // trial, FP64 return mapping (tensile cut-off, then shear return with
@@ -663,8 +775,10 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
     if (n <= 0) return;
     if (variant == 0)
         k_interaction_direct<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, rt, beg, end, n, nbr);
-    else
+    else if (variant == 1)
         k_interaction_lds<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, rt, beg, end, n, nbr);
+    else
+        k_interaction_unrolled<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, rt, beg, end, n, nbr);
 }
 
 void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates& rt, const int32_t* beg,
