@@ -252,6 +252,13 @@ lq_status lq_engine_run_info(const lq_engine* eng, int32_t run_id, int64_t* step
  * arrays, np rows; posd double[3*np] optional).  Synchronous. */
 lq_status lq_engine_download(lq_engine* eng, int32_t run_id, lq_host_particles* out, double* posd);
 
+/* Collects the per-run failure flags the corrector raises on the device when
+ * a particle's new state is not finite (NaN/Inf).  A failed run stops
+ * stepping at once; the other runs are unaffected.  Synchronous; nfailed
+ * (nullable) receives the number of failed resident runs. */
+lq_status lq_engine_poll(lq_engine* eng, int32_t* nfailed);
+lq_status lq_engine_run_failed(const lq_engine* eng, int32_t run_id, int32_t* failed);
+
 /* Sum over resident runs: material particles, all particles, cells. */
 lq_status lq_engine_totals(const lq_engine* eng, int64_t* npm, int64_t* np, int64_t* ncell, int32_t* nruns);
 
@@ -275,6 +282,49 @@ lq_status lq_engine_profile(lq_engine* eng, int32_t nsteps, double* ms_stage5);
  * 2 = LDS lists with a 4-wide unrolled candidate scan. */
 lq_status lq_set_interaction_variant(int32_t variant);
 int32_t lq_get_interaction_variant(void);
+
+/* ----------------------------------------------------------------- server
+ * Continuous batching of independent runs (no counterpart in the reference,
+ * which runs one simulation per process, src/loquat.cu:216-475).  Requests
+ * queue in arrival order; each tick admits the queue head while its device
+ * footprint fits the HBM budget and the capacities (strict FIFO), advances all
+ * resident runs steps_per_tick steps in shared launches, then evicts finished
+ * and failed runs.  Each run's result is bit-identical to running it alone. */
+
+typedef struct lq_server lq_server;
+
+typedef struct lq_server_config {
+    int64_t np_cap;           /* engine particle capacity (sum over resident runs) */
+    int64_t ncell_cap;        /* engine cell capacity */
+    int32_t max_runs;         /* resident runs at most */
+    int64_t hbm_budget;       /* bytes resident runs may occupy (lq_engine_bytes_per_run) */
+    int32_t steps_per_tick;   /* steps between admission/eviction points */
+    int32_t fake_backend;     /* 1: no GPU, runs advance on the host (scheduler tests) */
+    int32_t keep_results;     /* 1: download each finished run's state */
+} lq_server_config;
+
+typedef enum lq_request_state {
+    LQ_REQ_QUEUED = 0,
+    LQ_REQ_RUNNING = 1,
+    LQ_REQ_DONE = 2,
+    LQ_REQ_FAILED = 3,        /* non-finite state: evicted, others unaffected */
+    LQ_REQ_REJECTED = 4       /* larger than the budget or the capacities */
+} lq_request_state;
+
+lq_status lq_server_create(const lq_server_config* cfg, lq_server** out);
+lq_status lq_server_destroy(lq_server* srv);
+/* Queues a run (host arrays are copied).  request_id is set even when the
+ * request is rejected (state LQ_REQ_REJECTED). */
+lq_status lq_server_submit(lq_server* srv, const lq_run_input* in, int32_t* request_id);
+/* One admission / step / eviction round.  finished_ids receives up to cap ids
+ * of requests that completed or failed in this tick; nfinished the count. */
+lq_status lq_server_tick(lq_server* srv, int32_t* finished_ids, int32_t cap, int32_t* nfinished);
+lq_status lq_server_request_info(const lq_server* srv, int32_t request_id, int32_t* state, int64_t* steps_done,
+                                 int64_t* nsteps, int64_t* bytes);
+/* Final state of a finished request in input order (keep_results = 1). */
+lq_status lq_server_result(lq_server* srv, int32_t request_id, lq_host_particles* out, double* posd);
+lq_status lq_server_stats(const lq_server* srv, int32_t* queued, int32_t* running, int64_t* bytes_in_use,
+                          int64_t* npm_running);
 
 /* Build/version string. */
 const char* lq_version(void);

@@ -80,6 +80,14 @@ class LqRunInput(C.Structure):
     _fields_ = [("par", LqParams), ("vw", LqVw), ("host", LqHostParticles), ("posd", vp), ("nsteps", C.c_int64)]
 
 
+class LqServerConfig(C.Structure):
+    _fields_ = [("np_cap", C.c_int64), ("ncell_cap", C.c_int64), ("max_runs", i32), ("hbm_budget", C.c_int64),
+                ("steps_per_tick", i32), ("fake_backend", i32), ("keep_results", i32)]
+
+
+REQ_STATE = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "rejected"}
+
+
 class LqError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"lq_status {status}: {msg}")
@@ -126,6 +134,16 @@ _PROTOS = {
     "lq_engine_stream": (vp, [vp]),
     "lq_engine_set_graphs": (i32, [vp, i32]),
     "lq_engine_profile": (i32, [vp, i32, C.POINTER(f64)]),
+    "lq_engine_poll": (i32, [vp, C.POINTER(i32)]),
+    "lq_engine_run_failed": (i32, [vp, i32, C.POINTER(i32)]),
+    "lq_server_create": (i32, [C.POINTER(LqServerConfig), C.POINTER(vp)]),
+    "lq_server_destroy": (i32, [vp]),
+    "lq_server_submit": (i32, [vp, C.POINTER(LqRunInput), C.POINTER(i32)]),
+    "lq_server_tick": (i32, [vp, C.POINTER(i32), i32, C.POINTER(i32)]),
+    "lq_server_request_info": (i32, [vp, i32, C.POINTER(i32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_int64)]),
+    "lq_server_result": (i32, [vp, i32, C.POINTER(LqHostParticles), vp]),
+    "lq_server_stats": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "lq_set_interaction_variant": (i32, [i32]),
     "lq_get_interaction_variant": (i32, []),
     "lq_version": (C.c_char_p, []),

@@ -27,7 +27,7 @@ ARCH = os.environ.get("LOQUAT_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["lq_grid.hip", "lq_physics.hip", "lq_engine.hip"]
-CXX_SOURCES = ["lq_io.cpp"]
+CXX_SOURCES = ["lq_io.cpp", "lq_server.cpp"]
 
 # Packed FP32 instructions are disabled for all device code.
 HIP_FLAGS = [

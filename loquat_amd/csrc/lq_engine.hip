@@ -314,6 +314,7 @@ struct RunSlot {
     int32_t id;
     RunDesc desc;  // host mirror; dynamic fields advanced in lockstep with k_step_end
     int64_t nsteps;
+    int32_t failed;  // non-finite state seen by the corrector (lq_engine_poll)
 };
 
 struct Buffers {
@@ -323,6 +324,7 @@ struct Buffers {
     tensor2d* T;
     int32_t* run_of;
     RunDesc* desc;
+    int32_t* fail;  // per-run flag, raised on the device by the corrector
     GridBuffers grid;
 };
 
@@ -426,7 +428,7 @@ void enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
         }
         {
             StageTimer t(prof, s, PG_INTEG);
-            launch_integrate_table(corr, b.desc, b.run_of, b.S[c], corr ? b.I[1] : b.I[0], b.R, n, s);
+            launch_integrate_table(corr, b.desc, b.run_of, b.S[c], corr ? b.I[1] : b.I[0], b.R, n, b.fail, s);
         }
     }
     if (shepard) {
@@ -439,7 +441,7 @@ void enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
         StageTimer t(prof, s, PG_SHEP);
         launch_shepard_table(b.desc, b.run_of, b.S[c], b.T, b.S[c].str, b.grid.cell_beg, b.grid.cell_end, n, s);
     }
-    launch_step_end(b.desc, nr, s);
+    launch_step_end(b.desc, nr, b.fail, s);
 }
 
 // Host mirror of k_step_begin's Shepard test over all active runs.
@@ -464,9 +466,15 @@ void host_step_end(lq_engine* e) {
 
 lq_status upload_descs(lq_engine* e) {
     std::vector<RunDesc> h(e->runs.size());
-    for (size_t i = 0; i < e->runs.size(); ++i) h[i] = e->runs[i].desc;
-    if (!h.empty())
+    std::vector<int32_t> f(e->runs.size());
+    for (size_t i = 0; i < e->runs.size(); ++i) {
+        h[i] = e->runs[i].desc;
+        f[i] = e->runs[i].failed;
+    }
+    if (!h.empty()) {
         LQ_HIP_TRY(hipMemcpyAsync(e->b.desc, h.data(), h.size() * sizeof(RunDesc), hipMemcpyHostToDevice, e->stream));
+        LQ_HIP_TRY(hipMemcpyAsync(e->b.fail, f.data(), f.size() * 4, hipMemcpyHostToDevice, e->stream));
+    }
     LQ_HIP_TRY(hipStreamSynchronize(e->stream));
     return LQ_OK;
 }
@@ -554,7 +562,8 @@ extern "C" lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t
     }
     ok = ok && dmalloc(b.R.accad, np_cap, e->owned) && dmalloc(b.R.spin, np_cap, e->owned) &&
          dmalloc(b.R.acestn, np_cap, e->owned) && dmalloc(b.T, np_cap, e->owned) &&
-         dmalloc(b.run_of, np_cap, e->owned) && dmalloc(b.desc, max_runs, e->owned);
+         dmalloc(b.run_of, np_cap, e->owned) && dmalloc(b.desc, max_runs, e->owned) &&
+         dmalloc(b.fail, max_runs, e->owned);
     if (!ok || alloc_grid(b.grid, np_cap, ncell_cap) != LQ_OK) {
         for (void* p : e->owned) (void)hipFree(p);
         (void)hipStreamDestroy(e->stream);
@@ -659,6 +668,7 @@ extern "C" lq_status lq_engine_add_run(lq_engine* e, const lq_run_input* in, int
     slot.nsteps = in->nsteps > 0 ? in->nsteps : steps_until((double)par.time, (double)par.time_max, (double)par.dt);
     slot.desc.nsteps = (int32_t)std::min<int64_t>(slot.nsteps, 0x7fffffff);
     slot.desc.active = slot.desc.nsteps > 0 ? 1 : 0;
+    slot.failed = 0;
     const DevState& S = e->b.S[e->cur];
     const int64_t o = e->np_total;
     std::vector<int32_t> orig(np), run_of(np, (int32_t)e->runs.size());
@@ -802,6 +812,41 @@ extern "C" lq_status lq_engine_run_info(const lq_engine* e, int32_t run_id, int6
     if (time) *time = d.time;
     if (finished) *finished = d.active ? 0 : 1;
     if (np) *np = d.np;
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_engine_poll(lq_engine* e, int32_t* nfailed) {
+    if (!e) {
+        set_error("null engine");
+        return LQ_ERR_INVALID_ARG;
+    }
+    LQ_HIP_TRY(hipStreamSynchronize(e->stream));
+    const size_t nr = e->runs.size();
+    std::vector<int32_t> f(nr, 0);
+    if (nr) LQ_HIP_TRY(hipMemcpy(f.data(), e->b.fail, nr * 4, hipMemcpyDeviceToHost));
+    int32_t total = 0;
+    for (size_t i = 0; i < nr; ++i) {
+        if (f[i] && !e->runs[i].failed) {
+            e->runs[i].failed = 1;
+            e->runs[i].desc.active = 0;
+        }
+        total += e->runs[i].failed;
+    }
+    if (nfailed) *nfailed = total;
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_engine_run_failed(const lq_engine* e, int32_t run_id, int32_t* failed) {
+    if (!e || !failed) {
+        set_error("lq_engine_run_failed: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    const int i = find_run(e, run_id);
+    if (i < 0) {
+        set_error("unknown run id");
+        return LQ_ERR_INVALID_ARG;
+    }
+    *failed = e->runs[i].failed;
     return LQ_OK;
 }
 

@@ -87,8 +87,9 @@ void launch_interaction_table(const RunDesc* d, const int32_t* run_of, const Dev
                               hipStream_t s);
 void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, const DevIni& ini,
                           const DevRates& rt, int n, hipStream_t s);
+// fail: per-run flags raised by the corrector on a non-finite state (nullable).
 void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
-                            const DevIni& ini, const DevRates& rt, int n, hipStream_t s);
+                            const DevIni& ini, const DevRates& rt, int n, int32_t* fail, hipStream_t s);
 // Shepard filter: reads stress from str_src, writes str_dst (walls and runs
 // not due are copied through).
 void launch_shepard_one(const RunDesc& d, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,
@@ -97,7 +98,7 @@ void launch_shepard_table(const RunDesc* d, const int32_t* run_of, const DevStat
                           tensor2d* str_dst, const int32_t* beg, const int32_t* end, int n, hipStream_t s);
 // Per-run step bookkeeping on the device.
 void launch_step_begin(RunDesc* d, int nruns, hipStream_t s);
-void launch_step_end(RunDesc* d, int nruns, hipStream_t s);
+void launch_step_end(RunDesc* d, int nruns, const int32_t* fail, hipStream_t s);
 // Copies material stress back (reference's D2D copy, src/field_smoothing.cu:144).
 void launch_copy_tensor(const tensor2d* src, tensor2d* dst, int n, hipStream_t s);
 

@@ -649,11 +649,15 @@ __global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni in
 }
 
 // Corrector (src/predictor_corrector.cu:256-307): midpoint update from ini.
+// fail (nullable, one flag per run) is raised when a particle's new state is
+// not finite, so a diverging run can be stopped without touching the others.
 template <class DP>
-__global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni ini, DevRates rt, int n) {
+__global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni ini, DevRates rt, int n,
+                                                   int32_t* __restrict__ fail) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const RunDesc& d = dp.get(dp.run(k));
+    const int run = dp.run(k);
+    const RunDesc& d = dp.get(run);
     if ((k - d.p_off) < d.npb || !d.active) return;
     const float dt = d.par.dt;
     const double dtd = (double)dt;
@@ -674,6 +678,10 @@ __global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni in
     float ep = epvr.x, voidr = epvr.y;
     tensor2d s = ini.str[k];
     constitutive(d, dt, rt.spin[k], D, stn, p, ep, voidr, s);
+    if (fail) {
+        const float chk = (vr.x + vr.y + vr.z + vr.w) + (s.xx + s.yy + s.zz) + (s.xy + s.xz + s.yz);
+        if (!isfinite(chk) || !isfinite(x + y + z)) fail[run] = 1;
+    }
 
     st.xd[k] = x;
     st.yd[k] = y;
@@ -744,10 +752,14 @@ __global__ void k_step_begin(RunDesc* d, int nruns) {
     d[r].shepard_due = (d[r].active && f != 0 && step % f == 0 && step != 0) ? 1 : 0;
 }
 
-__global__ void k_step_end(RunDesc* d, int nruns) {
+__global__ void k_step_end(RunDesc* d, int nruns, const int32_t* __restrict__ fail) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nruns) return;
     if (!d[r].active) return;
+    if (fail && fail[r]) {
+        d[r].active = 0;
+        return;
+    }
     d[r].time += (double)d[r].par.dt;
     d[r].step += 1;
     if (d[r].step >= d[r].nsteps) d[r].active = 0;
@@ -795,15 +807,15 @@ void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, 
                           const DevRates& rt, int n, hipStream_t s) {
     if (n <= 0) return;
     if (corrector)
-        k_corrector<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, n);
+        k_corrector<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, n, nullptr);
     else
         k_predictor<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, n);
 }
 void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
-                            const DevIni& ini, const DevRates& rt, int n, hipStream_t s) {
+                            const DevIni& ini, const DevRates& rt, int n, int32_t* fail, hipStream_t s) {
     if (n <= 0) return;
     if (corrector)
-        k_corrector<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, n);
+        k_corrector<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, n, fail);
     else
         k_predictor<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, n);
 }
@@ -822,8 +834,8 @@ void launch_shepard_table(const RunDesc* d, const int32_t* run_of, const DevStat
 void launch_step_begin(RunDesc* d, int nruns, hipStream_t s) {
     if (nruns > 0) k_step_begin<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns);
 }
-void launch_step_end(RunDesc* d, int nruns, hipStream_t s) {
-    if (nruns > 0) k_step_end<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns);
+void launch_step_end(RunDesc* d, int nruns, const int32_t* fail, hipStream_t s) {
+    if (nruns > 0) k_step_end<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns, fail);
 }
 void launch_copy_tensor(const tensor2d* src, tensor2d* dst, int n, hipStream_t s) {
     if (n > 0) k_copy_tensor<<<blocks_for(n, 256), 256, 0, s>>>(src, dst, n);
