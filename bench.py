@@ -31,6 +31,7 @@ FLOPS_PER_PAIR = 80            # algorithmic FP32 flops per neighbour pair
 BYTES_READ_PER_PARTICLE = 64   # x (FP64, 24) + v (12) + rho (4) + stress (24)
 BYTES_WRITE_PER_MATERIAL = 52  # a (12) + drho (4) + spin (12) + strain rate (24)
 BYTES_PER_CELL = 8             # cell_beg + cell_end
+KERNEL_NAMES = {0: "k_interaction_direct", 1: "k_interaction_lds", 2: "k_interaction_unrolled"}
 
 
 def log(msg: str) -> None:
@@ -94,6 +95,174 @@ def pair_count(case) -> int:
     return n
 
 
+def _pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, float), q)) if len(xs) else None
+
+
+def ensemble_section(nruns: int, steps: int, warm: int, seed: int = 0) -> dict:
+    """256 independent 3D columns (npm log-uniform 2k..60k, aspect U[0.5,4],
+    phi U[20,40] deg, c U[0,5] kPa): all runs in one engine (one launch per
+    stage, graph-replayed) against the plain loop (the same engine code, one
+    run at a time).  Bounded sample: `steps` timed steps per run."""
+    import torch
+
+    from loquat_amd import cases
+    from loquat_amd.device import Engine
+
+    runs = cases.ensemble(nruns=nruns, seed=seed, nsteps=steps + warm)
+    npm_tot = sum(c.npm for c in runs)
+    # batched
+    eng = Engine(sum(c.np for c in runs), sum(c.derived().ncell for c in runs), len(runs))
+    for c in runs:
+        eng.add(c)
+    eng.step(warm)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step(steps)
+    eng.sync()
+    t_batch = time.perf_counter() - t0
+    eng.close()
+    torch.cuda.empty_cache()
+    # plain loop: one run at a time, each a batch of one
+    big = max(runs, key=lambda c: c.np)
+    eng = Engine(big.np, max(c.derived().ncell for c in runs), 1)
+    per_run, completion, clock = [], [], 0.0
+    for c in runs:
+        rid = eng.add(c)
+        eng.step(warm)
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.step(steps)
+        eng.sync()
+        dt = time.perf_counter() - t0
+        eng.remove(rid)
+        per_run.append(dt)
+        clock += dt
+        completion.append(clock)
+    eng.close()
+    batched = npm_tot * steps / t_batch / 1e6
+    plain = npm_tot * steps / sum(per_run) / 1e6
+    return {
+        "workload": f"{nruns} independent 3D column collapses, npm log-uniform 2k..60k (sum {npm_tot}), "
+                    f"DP phi U[20,40] deg, c U[0,5] kPa, Shepard every 20 steps",
+        "sample": f"{steps} timed steps per run (+{warm} warm-up) of the 5,000",
+        "npm_total": npm_tot,
+        "batched_M_particle_steps_s": round(batched, 3),
+        "plain_loop_M_particle_steps_s": round(plain, 3),
+        "speedup_vs_plain_loop": round(batched / plain, 3),
+        "batched_completion_s": {"p50": round(t_batch, 4), "p99": round(t_batch, 4)},
+        "plain_loop_completion_s": {"p50": round(_pct(completion, 50), 4), "p99": round(_pct(completion, 99), 4)},
+    }
+
+
+def served_section(nreq: int, step_scale: float, seed: int = 0) -> dict:
+    """Poisson arrivals (20 runs/s) of column (Drucker-Prager) and embankment
+    (von Wolffersdorff) runs, 50/50, material particles log-normal in
+    [5k, 500k], 2,000-8,000 steps scaled by step_scale; admitted by HBM
+    footprint into one engine and batched every step (C++ request loop).
+    The plain loop serves the same arrivals one at a time, FIFO."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from loquat_amd import cases
+    from loquat_amd.device import Engine
+    from loquat_amd.serve import Server
+
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / 20.0, nreq))
+    reqs = []
+    for i in range(nreq):
+        npm = float(np.clip(math.exp(rng.normal(math.log(3e4), 1.0)), 5e3, 5e5))
+        steps = max(1, int(round(rng.integers(2000, 8001) * step_scale)))
+        if rng.random() < 0.5:
+            aspect = rng.uniform(0.5, 4.0)
+            nw = max(2, round((npm / aspect) ** (1 / 3)))
+            w, hh = nw * 0.005, max(2, round(aspect * nw)) * 0.005
+            c = cases.column(name=f"req{i}", dx=0.005, col=(w, w, hh), box=(w + 2 * hh, w), quasi2d=False,
+                             phi=rng.uniform(20, 40), c=rng.uniform(0, 5e3), shepard=20, nsteps=steps, seed=i)
+        else:
+            c = cases.slope_vw(npm_target=npm, nsteps=steps, seed=i, depth=0.1)
+        reqs.append((arrivals[i], c, steps))
+    np_cap = sum(c.np for _, c, _ in reqs)
+    nc_cap = sum(c.derived().ncell for _, c, _ in reqs)
+    srv = Server(np_cap, nc_cap, 256, 288 * 10**9, steps_per_tick=25)
+    done_at, ids = {}, {}
+    t0 = time.perf_counter()
+    nxt = 0
+    while len(done_at) < nreq:
+        now = time.perf_counter() - t0
+        while nxt < nreq and reqs[nxt][0] <= now:
+            ids[srv.submit(reqs[nxt][1], reqs[nxt][2])] = nxt
+            nxt += 1
+        if srv.stats()["running"] == 0 and srv.stats()["queued"] == 0:
+            if nxt < nreq:
+                time.sleep(max(0.0, min(0.01, reqs[nxt][0] - now)))
+            continue
+        for rid in srv.tick():
+            done_at[ids[rid]] = time.perf_counter() - t0
+    makespan = time.perf_counter() - t0
+    srv.close()
+    torch.cuda.empty_cache()
+    lat = [done_at[i] - reqs[i][0] for i in range(nreq)]
+    work = sum(c.npm * s for _, c, s in reqs)
+    # plain loop: FIFO, one run at a time on the same engine code
+    big = max(reqs, key=lambda r: r[1].np)[1]
+    eng = Engine(big.np, max(c.derived().ncell for _, c, _ in reqs), 1)
+    clock, lat_plain, busy = 0.0, [], 0.0
+    for a, c, s in reqs:
+        rid = eng.add(c, s)
+        eng.sync()
+        t1 = time.perf_counter()
+        eng.step(s)
+        eng.sync()
+        dt = time.perf_counter() - t1
+        eng.remove(rid)
+        busy += dt
+        clock = max(clock, a) + dt
+        lat_plain.append(clock - a)
+    eng.close()
+    return {
+        "workload": f"{nreq} requests, Poisson 20/s, column DP / slope von Wolffersdorff 50/50, "
+                    f"npm log-normal clipped [5k,500k], steps U[2000,8000] x {step_scale}",
+        "latency_s": {"p50": round(_pct(lat, 50), 4), "p99": round(_pct(lat, 99), 4)},
+        "plain_loop_latency_s": {"p50": round(_pct(lat_plain, 50), 4), "p99": round(_pct(lat_plain, 99), 4)},
+        "aggregate_M_particle_steps_s": round(work / makespan / 1e6, 3),
+        "plain_loop_M_particle_steps_s": round(work / busy / 1e6, 3),
+        "makespan_s": round(makespan, 3),
+    }
+
+
+def slope_section(steps: int, npm_target: float = 4.0e6) -> dict:
+    """30-degree embankment, ~4 M material particles, von Wolffersdorff with a
+    seeded +-5% void-ratio field."""
+    import torch
+
+    from loquat_amd import cases
+    from loquat_amd.device import Engine
+
+    c = cases.slope_vw(npm_target=npm_target, nsteps=steps + 5)
+    eng = Engine(c.np, c.derived().ncell, 1)
+    eng.add(c)
+    eng.step(5)
+    eng.sync()
+    s = torch.cuda.ExternalStream(eng.stream())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    eng.step(steps)
+    e1.record(s)
+    eng.sync()
+    ms = e0.elapsed_time(e1)
+    eng.close()
+    torch.cuda.empty_cache()
+    return {"workload": f"30-degree embankment 1.0 x 3.0 x 0.5 m, npm={c.npm}, np={c.np}, "
+                        f"dx={c.meta['dx'] * 1e3:.2f} mm, von Wolffersdorff",
+            "M_particle_steps_s": round(c.npm * steps / (ms * 1e-3) / 1e6, 3),
+            "ms_per_step": round(ms / steps, 4), "steps": steps}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,7 +271,15 @@ def main() -> None:
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variant", type=int, default=1, help="interaction kernel variant")
+    ap.add_argument("--variant", type=int, default=2, help="interaction kernel variant")
+    ap.add_argument("--ensemble-runs", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=200)
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-step-scale", type=float, default=0.05)
+    ap.add_argument("--slope-steps", type=int, default=20)
+    ap.add_argument("--no-extra", action="store_true", help="skip the ensemble / served / slope sections")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="after the run, 5 device copies of MB MiB (reference kernel for HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the SPH step of one run lives on one device "
@@ -185,7 +362,7 @@ def main() -> None:
             "steps_done": info["steps"],
         },
         "roofline": {
-            "kernel": "k_interaction_lds" if a.variant == 1 else "k_interaction_direct",
+            "kernel": KERNEL_NAMES[a.variant],
             "bound": "hbm",
             "achieved": round(gbs, 2),
             "peak": HBM_PEAK_GBS,
@@ -206,6 +383,27 @@ def main() -> None:
                               zip(("grid", "adami", "interaction", "integrate", "shepard"), stage)},
         "cpu_baseline": None,
     }
+    torch.cuda.empty_cache()
+    if a.calibrate_copy > 0:
+        n = a.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["calibration_copy_bytes"] = n * 4
+        del src, dst
+        torch.cuda.empty_cache()
+    if not a.no_extra:
+        t = time.perf_counter()
+        out["ensemble"] = ensemble_section(a.ensemble_runs, a.ensemble_steps, 10)
+        log(f"ensemble: {out['ensemble']} ({time.perf_counter() - t:.1f} s)")
+        t = time.perf_counter()
+        out["served"] = served_section(a.serve_requests, a.serve_step_scale)
+        log(f"served: {out['served']} ({time.perf_counter() - t:.1f} s)")
+        t = time.perf_counter()
+        out["slope_vw"] = slope_section(a.slope_steps)
+        log(f"slope: {out['slope_vw']} ({time.perf_counter() - t:.1f} s)")
     if not a.no_cpu:
         log(f"CPU baseline: FP64 oracle, {a.cpu_steps} steps ...")
         out["cpu_baseline"] = cpu_baseline(cases.column_3d(nsteps=a.cpu_steps + 1), a.cpu_steps)

@@ -19,6 +19,12 @@ class Server:
         abi.check(abi.lib().lq_server_create(C.byref(cfg), C.byref(self.h)))
         self._buf = (C.c_int32 * 4096)()
 
+    @staticmethod
+    def footprint(case: Case) -> int:
+        """Device bytes the run occupies once admitted."""
+        p = case.derived()
+        return abi.lib().lq_engine_bytes_per_run(p.np, p.ncell)
+
     def submit(self, case: Case, nsteps: int | None = None) -> int:
         ri = case.run_input(nsteps)
         rid = C.c_int32()
