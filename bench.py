@@ -31,7 +31,8 @@ FLOPS_PER_PAIR = 80            # algorithmic FP32 flops per neighbour pair
 BYTES_READ_PER_PARTICLE = 64   # x (FP64, 24) + v (12) + rho (4) + stress (24)
 BYTES_WRITE_PER_MATERIAL = 52  # a (12) + drho (4) + spin (12) + strain rate (24)
 BYTES_PER_CELL = 8             # cell_beg + cell_end
-KERNEL_NAMES = {0: "k_interaction_direct", 1: "k_interaction_lds", 2: "k_interaction_unrolled"}
+KERNEL_NAMES = {0: "k_interaction_direct", 1: "k_interaction_lds", 2: "k_interaction_unrolled",
+                3: "k_interaction_v3"}
 
 
 def log(msg: str) -> None:
@@ -271,7 +272,7 @@ def main() -> None:
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variant", type=int, default=2, help="interaction kernel variant")
+    ap.add_argument("--variant", type=int, default=3, help="interaction kernel variant")
     ap.add_argument("--ensemble-runs", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--serve-requests", type=int, default=48)

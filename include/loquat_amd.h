@@ -279,7 +279,8 @@ lq_status lq_engine_profile(lq_engine* eng, int32_t nsteps, double* ms_stage5);
 
 /* Interaction kernel variant for the engine and lq_particle_interaction:
  * 0 = per-thread direct loop, 1 = LDS-compacted neighbour lists,
- * 2 = LDS lists with a 4-wide unrolled candidate scan. */
+ * 2 = LDS lists with a 4-wide unrolled candidate scan,
+ * 3 = variant 2 with offset loads, XCD-contiguous blocks (default). */
 lq_status lq_set_interaction_variant(int32_t variant);
 int32_t lq_get_interaction_variant(void);
 

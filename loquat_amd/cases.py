@@ -176,6 +176,18 @@ def column_2d_small(nsteps: int = 20000, seed: int = 0) -> Case:
                   shepard=0, nsteps=nsteps)
 
 
+def column_2d_slab(nsteps: int = 20000, seed: int = 0, ny: int = 8) -> Case:
+    """The plane-strain column of column_2d_small as a slab ny particles thick
+    between two y side walls (the reference's plane example is a thick slab
+    too: examples/sand_collapse_plane.xml holds 46 particles across y).  A
+    one-particle slab under the always-3-D kernel is not supported by its
+    floor; this slab is, so it carries the full-run comparison."""
+    dx = 0.002
+    return column(name="column_2d_slab", dx=dx, col=(0.1, ny * dx, 0.2), box=(0.5, ny * dx), quasi2d=False,
+                  jitter=0.01, seed=seed, rho0=2650.0, E=0.84e6, nu=0.3, phi=19.8, c=0.0, psi=0.0,
+                  shepard=0, nsteps=nsteps)
+
+
 def column_3d(nsteps: int = 5000, dx: float = 0.005) -> Case:
     """3D column 0.4 x 0.4 x 0.8 m at dx = 5 mm (80 x 80 x 160 = 1,024,000
     material particles) in a 2.0 x 0.4 m box; floor, x = 0 and both side walls;

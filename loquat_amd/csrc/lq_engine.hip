@@ -25,7 +25,7 @@ namespace lq {
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 
-static int g_variant = 1;
+static int g_variant = 3;
 int interaction_variant() { return g_variant; }
 
 // Smallest FP32 value T with sqrtf(T) * i_h >= 2 (both correctly rounded):
@@ -145,8 +145,8 @@ extern "C" lq_status lq_neighbour_r2_cut(const lq_params* par, float* r2_cut) {
 }
 
 extern "C" lq_status lq_set_interaction_variant(int32_t v) {
-    if (v < 0 || v > 2) {
-        set_error("interaction variant must be 0, 1 or 2");
+    if (v < 0 || v > 3) {
+        set_error("interaction variant must be 0, 1, 2 or 3");
         return LQ_ERR_INVALID_ARG;
     }
     g_variant = v;
@@ -319,7 +319,7 @@ struct RunSlot {
 
 struct Buffers {
     DevState S[2];
-    DevIni I[2];
+    DevIni I;  // written by the predictor, read by the corrector through the sort permutation
     DevRates R;
     tensor2d* T;
     int32_t* run_of;
@@ -411,10 +411,9 @@ void enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
         {
             StageTimer t(prof, s, PG_GRID);
             enqueue_build(e, b.S[c], s);
-            if (corr)
-                launch_permute(b.grid.vals[b.grid.sorted_slot], n, b.S[c], b.S[1 - c], &b.I[0], &b.I[1], nullptr, s);
-            else
-                launch_permute(b.grid.vals[b.grid.sorted_slot], n, b.S[c], b.S[1 - c], nullptr, nullptr, nullptr, s);
+            // The ini set stays in the predictor-half order: the corrector
+            // reads it through this build's permutation.
+            launch_permute(b.grid.vals[b.grid.sorted_slot], n, b.S[c], b.S[1 - c], nullptr, nullptr, nullptr, s);
         }
         c = 1 - c;
         {
@@ -428,7 +427,8 @@ void enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
         }
         {
             StageTimer t(prof, s, PG_INTEG);
-            launch_integrate_table(corr, b.desc, b.run_of, b.S[c], corr ? b.I[1] : b.I[0], b.R, n, b.fail, s);
+            launch_integrate_table(corr, b.desc, b.run_of, b.S[c], b.I, b.R, n,
+                                   corr ? b.grid.vals[b.grid.sorted_slot] : nullptr, b.fail, s);
         }
     }
     if (shepard) {
@@ -526,9 +526,9 @@ lq_status relayout(lq_engine* e) {
 }  // namespace
 
 extern "C" int64_t lq_engine_bytes_per_run(int64_t np, int64_t ncell) {
-    // 2 states (124 B) + 2 ini (96 B) + rates (56 B) + Shepard scratch (24 B)
+    // 2 states (124 B) + ini (96 B) + rates (56 B) + Shepard scratch (24 B)
     // + run id (4 B) + sort keys/values (16 B) + histograms, + table 8 B/cell.
-    const int64_t per_particle = 2 * 124 + 2 * 96 + 56 + 24 + 4 + 16 + 1;
+    const int64_t per_particle = 2 * 124 + 96 + 56 + 24 + 4 + 16 + 1;
     return np * per_particle + ncell * 8 + (int64_t)sizeof(RunDesc);
 }
 
@@ -555,7 +555,9 @@ extern "C" lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t
              dmalloc(S.velrhop, np_cap, e->owned) && dmalloc(S.idep, np_cap, e->owned) &&
              dmalloc(S.str, np_cap, e->owned) && dmalloc(S.stn, np_cap, e->owned) &&
              dmalloc(S.orig, np_cap, e->owned);
-        DevIni& I = b.I[i];
+    }
+    {
+        DevIni& I = b.I;
         ok = ok && dmalloc(I.xd, np_cap, e->owned) && dmalloc(I.yd, np_cap, e->owned) &&
              dmalloc(I.zd, np_cap, e->owned) && dmalloc(I.velrhop, np_cap, e->owned) &&
              dmalloc(I.epvr, np_cap, e->owned) && dmalloc(I.str, np_cap, e->owned) && dmalloc(I.stn, np_cap, e->owned);

@@ -87,9 +87,12 @@ void launch_interaction_table(const RunDesc* d, const int32_t* run_of, const Dev
                               hipStream_t s);
 void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, const DevIni& ini,
                           const DevRates& rt, int n, hipStream_t s);
-// fail: per-run flags raised by the corrector on a non-finite state (nullable).
+// ini_perm: the corrector reads particle k's ini entry at ini_perm[k] (the
+// last sort's permutation; nullable = identity).  fail: per-run flags raised
+// by the corrector on a non-finite state (nullable).
 void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
-                            const DevIni& ini, const DevRates& rt, int n, int32_t* fail, hipStream_t s);
+                            const DevIni& ini, const DevRates& rt, int n, const uint32_t* ini_perm, int32_t* fail,
+                            hipStream_t s);
 // Shepard filter: reads stress from str_src, writes str_dst (walls and runs
 // not due are copied through).
 void launch_shepard_one(const RunDesc& d, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,

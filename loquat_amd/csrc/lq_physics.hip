@@ -441,6 +441,171 @@ __global__ __launch_bounds__(256) void k_interaction_unrolled(DP dp, DevState st
     }
 }
 
+// Block index remapped so that each XCD works on one contiguous slice of the
+// sorted particles.  Workgroups are dealt to the 8 XCDs round-robin; without
+// the remap every XCD touches the whole domain and spatial neighbours are
+// fetched into all eight L2s.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int xcd = b & 7, idx = b >> 3;
+    const int q = nb >> 3, r = nb & 7;
+    return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+// 32-bit byte offset loads from a uniform base (saddr addressing: one shift
+// per load instead of 64-bit address arithmetic).
+template <class T>
+__device__ __forceinline__ T ldo(const T* __restrict__ base, uint32_t idx) {
+    return *(const T*)((const char*)base + (idx * (uint32_t)sizeof(T)));
+}
+
+// Pair body of variant 3: as pair_eval, with 32-bit offset loads and the
+// viscosity term evaluated branch-free.
+__device__ __forceinline__ void pair_eval3(const IPart& pi, uint32_t j, const DevState& st, const IConst& c,
+                                           IAcc& a) {
+    const float dx = (float)(pi.x - ldo(st.xd, j)), dy = (float)(pi.y - ldo(st.yd, j)),
+                dz = (float)(pi.z - ldo(st.zd, j));
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float q = fsqrt(r2) * c.i_h;
+    const float t = fmaxf(2.0f - q, 0.0f);
+    const float fac = c.cg * t * t * t;
+    const float gx = fac * dx, gy = fac * dy, gz = fac * dz;
+    const float4 vr = ldo(st.velrhop, j);
+    const tensor2d sj = ldo(st.str, j);
+    const float ir = frcp(vr.w);
+    const float dvx = vr.x - pi.vx, dvy = vr.y - pi.vy, dvz = vr.z - pi.vz;
+    const float ux = dvx * ir, uy = dvy * ir, uz = dvz * ir;
+    a.L[0] += ux * gx;
+    a.L[1] += ux * gy;
+    a.L[2] += ux * gz;
+    a.L[3] += uy * gx;
+    a.L[4] += uy * gy;
+    a.L[5] += uy * gz;
+    a.L[6] += uz * gx;
+    a.L[7] += uz * gy;
+    a.L[8] += uz * gz;
+    a.dvg += dvx * gx + dvy * gy + dvz * gz;
+    const float vdx = -(dvx * dx + dvy * dy + dvz * dz);
+    const float mu = c.h * fminf(vdx, 0.0f) * frcp(r2 + c.eta);
+    const float pij = -2.0f * c.alpha_cs * mu * frcp(pi.rho + vr.w);
+    const float ir2 = ir * ir;
+    const float sxx = pi.T.xx + sj.xx * ir2 - pij;
+    const float syy = pi.T.yy + sj.yy * ir2 - pij;
+    const float szz = pi.T.zz + sj.zz * ir2 - pij;
+    const float sxy = pi.T.xy + sj.xy * ir2;
+    const float sxz = pi.T.xz + sj.xz * ir2;
+    const float syz = pi.T.yz + sj.yz * ir2;
+    a.ax += sxx * gx + sxy * gy + sxz * gz;
+    a.ay += sxy * gx + syy * gy + syz * gz;
+    a.az += sxz * gx + syz * gy + szz * gz;
+}
+
+// Variant 3: variant 2's structure with fewer instructions per candidate
+// and per pair, and XCD-contiguous blocks.  The particle itself stays in its
+// candidate stream (it is always a hit when it lies inside the grid): all its
+// pair terms are exactly zero (x_i - x_i = 0 zeroes the kernel gradient and
+// v_i - v_i the velocity terms), so it changes no sum, and one is taken off
+// the neighbour count.
+template <class DP>
+__global__ __launch_bounds__(256) void k_interaction_v3(DP dp, DevState st, DevRates rt,
+                                                        const int32_t* __restrict__ beg,
+                                                        const int32_t* __restrict__ end, int n,
+                                                        int32_t* __restrict__ nbr) {
+    __shared__ int32_t lists[4][IC2_CAP][64];
+    const int k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int32_t(*my)[64] = lists[w];
+    bool mat = false;
+    const RunDesc* dptr = nullptr;
+    if (k < n) {
+        const RunDesc& d = dp.get(dp.run(k));
+        dptr = &d;
+        mat = (k - d.p_off) >= d.npb && d.active;
+    }
+    IConst c = {};
+    float r2cut = 0.f;
+    float4 ps = make_float4(0.f, 0.f, 0.f, 0.f);
+    IPart pi = {};
+    int cix = 0, ciy = 0, ciz = 0, nx = 0, ny = 0, nz = 0, nol = 0, cbase = 0;
+    if (mat) {
+        const RunDesc& d = *dptr;
+        c = make_iconst(d);
+        r2cut = d.r2_cut;
+        ps = st.pospres[k];
+        load_part(st, k, pi);
+        cix = cell_coord(ps.x, c.i_h);
+        ciy = cell_coord(ps.y, c.i_h);
+        ciz = cell_coord(ps.z, c.i_h);
+        nx = d.par.cell_num_x;
+        ny = d.par.cell_num_y;
+        nz = d.par.cell_num_z;
+        nol = d.par.ncell_one_layer;
+        cbase = d.cell_base;
+    }
+    const bool self_in = mat && cix >= 0 && cix < nx && ciy >= 0 && ciy < ny && ciz >= 0 && ciz < nz;
+    IAcc a = {};
+    int count = 0, cnt = 0;
+    const int czlo = max(ciz - 1, 0), czhi = min(ciz + 1, nz - 1);
+    const float4* __restrict__ pp = st.pospres;
+    for (int col = 0; col < 18; ++col) {
+        const int layer = col / 9, ox = (col % 9) / 3 - 1, oy = col % 3 - 1;
+        int j = 0, je = 0;
+        if (mat && czlo <= czhi) {
+            const int cx = cix + ox, cy = ciy + oy;
+            if (cx >= 0 && cx < nx && cy >= 0 && cy < ny) {
+                const int c0 = cbase + layer * nol + (cx * ny + cy) * nz;
+                int b0 = -1, e0 = -1;
+                for (int cz = czlo; cz <= czhi; ++cz) {
+                    const int b = beg[c0 + cz];
+                    if (b != -1) {
+                        if (b0 < 0) b0 = b;
+                        e0 = end[c0 + cz];
+                    }
+                }
+                if (b0 >= 0) {
+                    j = b0;
+                    je = e0;
+                }
+            }
+        }
+        const uint32_t jlast = (uint32_t)max(je - 1, 0);
+        while (true) {
+            while (__ballot(j < je) && !__ballot(cnt > IC2_CAP - IC2_U)) {
+                float4 q[IC2_U];
+#pragma unroll
+                for (int u = 0; u < IC2_U; ++u) q[u] = ldo(pp, min((uint32_t)(j + u), jlast));
+#pragma unroll
+                for (int u = 0; u < IC2_U; ++u) {
+                    const bool hit = (j + u < je) && r2_fp32(ps.x - q[u].x, ps.y - q[u].y, ps.z - q[u].z) < r2cut;
+                    my[cnt][lane] = j + u;
+                    cnt += hit ? 1 : 0;
+                }
+                j += IC2_U;
+            }
+            if (!__ballot(j < je)) break;
+            for (int e = 0; __ballot(e < cnt); ++e) {
+                if (e < cnt) pair_eval3(pi, (uint32_t)my[e][lane], st, c, a);
+            }
+            count += cnt;
+            cnt = 0;
+        }
+        if (__ballot(cnt > IC2_CAP - 3 * IC2_U)) {
+            for (int e = 0; __ballot(e < cnt); ++e) {
+                if (e < cnt) pair_eval3(pi, (uint32_t)my[e][lane], st, c, a);
+            }
+            count += cnt;
+            cnt = 0;
+        }
+    }
+    for (int e = 0; __ballot(e < cnt); ++e) {
+        if (e < cnt) pair_eval3(pi, (uint32_t)my[e][lane], st, c, a);
+    }
+    count += cnt;
+    if (mat) {
+        store_rates(*dptr, rt, k, a);
+        if (nbr) nbr[k] = count - (self_in ? 1 : 0);
+    }
+}
+
 // ------------------------------------------------------ constitutive models
 // Drucker-Prager (src/predictor_corrector.cu:30-105): FP32 Jaumann + elastic
 // trial, FP64 return mapping (tensile cut-off, then shear return with
@@ -649,34 +814,40 @@ __global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni in
 }
 
 // Corrector (src/predictor_corrector.cu:256-307): midpoint update from ini.
+// ini_perm (nullable): the ini set is still in the order of the previous
+// sort, and particle k's entry is ini[ini_perm[k]] -- the reference permutes
+// all five *_ini arrays instead (SortArraysAll, src/loquat.cu:200-213); the
+// values read are identical.
 // fail (nullable, one flag per run) is raised when a particle's new state is
 // not finite, so a diverging run can be stopped without touching the others.
 template <class DP>
 __global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni ini, DevRates rt, int n,
+                                                   const uint32_t* __restrict__ ini_perm,
                                                    int32_t* __restrict__ fail) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int run = dp.run(k);
     const RunDesc& d = dp.get(run);
     if ((k - d.p_off) < d.npb || !d.active) return;
+    const int q = ini_perm ? (int)ini_perm[k] : k;
     const float dt = d.par.dt;
     const double dtd = (double)dt;
     const float4 a = rt.accad[k];
-    const float4 v0 = ini.velrhop[k];
+    const float4 v0 = ini.velrhop[q];
     float4 vr;
     vr.x = v0.x + dt * a.x;
     vr.y = v0.y + dt * a.y;
     vr.z = v0.z + dt * a.z;
     vr.w = v0.w + dt * a.w;
-    const double x = 0.5 * ((double)vr.x + (double)v0.x) * dtd + ini.xd[k];
-    const double y = 0.5 * ((double)vr.y + (double)v0.y) * dtd + ini.yd[k];
-    const double z = 0.5 * ((double)vr.z + (double)v0.z) * dtd + ini.zd[k];
+    const double x = 0.5 * ((double)vr.x + (double)v0.x) * dtd + ini.xd[q];
+    const double y = 0.5 * ((double)vr.y + (double)v0.y) * dtd + ini.yd[q];
+    const double z = 0.5 * ((double)vr.z + (double)v0.z) * dtd + ini.zd[q];
     const tensor2d D = rt.acestn[k];
-    const tensor2d stn = tadd(ini.stn[k], tscale(D, dt));
+    const tensor2d stn = tadd(ini.stn[q], tscale(D, dt));
     float p = st.pospres[k].w;
-    const float2 epvr = ini.epvr[k];
+    const float2 epvr = ini.epvr[q];
     float ep = epvr.x, voidr = epvr.y;
-    tensor2d s = ini.str[k];
+    tensor2d s = ini.str[q];
     constitutive(d, dt, rt.spin[k], D, stn, p, ep, voidr, s);
     if (fail) {
         const float chk = (vr.x + vr.y + vr.z + vr.w) + (s.xx + s.yy + s.zz) + (s.xy + s.xz + s.yz);
@@ -789,8 +960,10 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
         k_interaction_direct<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, rt, beg, end, n, nbr);
     else if (variant == 1)
         k_interaction_lds<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, rt, beg, end, n, nbr);
-    else
+    else if (variant == 2)
         k_interaction_unrolled<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, rt, beg, end, n, nbr);
+    else
+        k_interaction_v3<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, rt, beg, end, n, nbr);
 }
 
 void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates& rt, const int32_t* beg,
@@ -807,15 +980,17 @@ void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, 
                           const DevRates& rt, int n, hipStream_t s) {
     if (n <= 0) return;
     if (corrector)
-        k_corrector<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, n, nullptr);
+        k_corrector<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, n, nullptr, nullptr);
     else
         k_predictor<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, n);
 }
 void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
-                            const DevIni& ini, const DevRates& rt, int n, int32_t* fail, hipStream_t s) {
+                            const DevIni& ini, const DevRates& rt, int n, const uint32_t* ini_perm, int32_t* fail,
+                            hipStream_t s) {
     if (n <= 0) return;
     if (corrector)
-        k_corrector<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, n, fail);
+        k_corrector<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, n, ini_perm,
+                                                                   fail);
     else
         k_predictor<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, n);
 }

@@ -337,7 +337,9 @@ void lqo_grid_build(const lq_params* par, int n, const double* x, const double* 
 }
 
 static void permute_arr(int n, const int32_t* perm, double* a, std::vector<double>& tmp) {
-    std::memcpy(tmp.data(), a, sizeof(double) * n);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n; ++k) tmp[k] = a[k];
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < n; ++k) a[k] = tmp[perm[k]];
 }
 

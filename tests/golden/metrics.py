@@ -1,11 +1,18 @@
-"""Order-independent full-run metrics of a column collapse."""
+"""Order-independent full-run metrics of a column collapse.
+
+A granular collapse is chaotic: perturbing the initial positions by 1e-9 m
+(far below FP32 resolution) moves the single farthest particle by several
+per cent.  These metrics use percentiles and means instead of extremes, so
+that two correct implementations can be compared at the 1 % level."""
 import numpy as np
 
 
 def collapse_metrics(case, posd) -> dict:
-    """runout: max x of material particles within 2 dx of the floor, measured
-    from the wall; profile: max height above the floor of the material in
-    x-bins of width 2 dx from the wall (NaN for empty bins)."""
+    """runout: 99th percentile of x (from the wall) of material particles
+    within 2 dx of the floor; com_x/com_z: centre of mass from wall/floor;
+    height: 99th percentile of the height above the floor; profile: 90th
+    percentile height in x-bins of width 10 dx from the wall (None for bins
+    holding fewer than 20 particles)."""
     dx = case.meta["dx"]
     x0, z0 = case.meta["wall_x"], case.meta["floor_z"]
     npb = case.par.npb
@@ -14,13 +21,13 @@ def collapse_metrics(case, posd) -> dict:
     ok = np.isfinite(x) & np.isfinite(z)
     x, z = x[ok], z[ok]
     base = z < 2 * dx
-    runout = float(x[base].max())
-    nb = int(np.ceil((x.max() + 1e-12) / (2 * dx)))
-    b = np.clip((x / (2 * dx)).astype(int), 0, nb - 1)
-    prof = np.full(nb, np.nan)
+    w = 10 * dx
+    nb = int(np.ceil(x.max() / w)) + 1
+    b = np.clip((x / w).astype(int), 0, nb - 1)
+    prof = []
     for i in range(nb):
         sel = z[b == i]
-        if sel.size:
-            prof[i] = sel.max()
-    return {"runout": runout, "height": float(z.max()), "bin": 2 * dx,
-            "profile": [None if np.isnan(v) else float(v) for v in prof]}
+        prof.append(float(np.percentile(sel, 90)) if sel.size >= 20 else None)
+    return {"runout": float(np.percentile(x[base], 99)), "com_x": float(x.mean()), "com_z": float(z.mean()),
+            "height": float(np.percentile(z, 99)), "bin": w, "profile": prof,
+            "below_floor": int(np.sum(z < 0)), "count": int(x.size)}

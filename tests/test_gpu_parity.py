@@ -147,7 +147,7 @@ def test_adami_parity(gpu):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("which", ["small3d", "c1"])
 def test_interaction_parity(gpu, variant, which):
     dev = _dev()
@@ -164,7 +164,7 @@ def test_interaction_parity(gpu, variant, which):
         h = rt.host()
         nb = nbr.cpu().numpy()
     finally:
-        abi.check(abi.lib().lq_set_interaction_variant(1))
+        abi.check(abi.lib().lq_set_interaction_variant(3))
     npb = p.npb
     assert np.array_equal(nb[npb:], onbr[npb:])
     assert nb[npb:].min() > 0
