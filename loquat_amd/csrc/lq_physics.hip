@@ -573,11 +573,18 @@ __global__ __launch_bounds__(256) void k_interaction_v3(DP dp, DevState st, DevR
                 float4 q[IC2_U];
 #pragma unroll
                 for (int u = 0; u < IC2_U; ++u) q[u] = ldo(pp, min((uint32_t)(j + u), jlast));
+                int hit[IC2_U];
 #pragma unroll
                 for (int u = 0; u < IC2_U; ++u) {
-                    const bool hit = (j + u < je) && r2_fp32(ps.x - q[u].x, ps.y - q[u].y, ps.z - q[u].z) < r2cut;
+                    // both tests evaluated, combined without a branch
+                    const int close = r2_fp32(ps.x - q[u].x, ps.y - q[u].y, ps.z - q[u].z) < r2cut ? 1 : 0;
+                    const int inside = (j + u < je) ? 1 : 0;
+                    hit[u] = close & inside;
+                }
+#pragma unroll
+                for (int u = 0; u < IC2_U; ++u) {
                     my[cnt][lane] = j + u;
-                    cnt += hit ? 1 : 0;
+                    cnt += hit[u];
                 }
                 j += IC2_U;
             }
