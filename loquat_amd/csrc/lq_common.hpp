@@ -53,6 +53,16 @@ struct DescTable {
     __device__ __forceinline__ const RunDesc& get(int r) const { return d[r]; }
 };
 
+// Maps a compact thread index t (over the material particles, or over the
+// walls, of all resident runs) to a particle index: idx[t] when a table is
+// given, else base + t (one run).  Launching over exactly the particles a
+// stage works on keeps every XCD's slice of blocks equally loaded.
+struct IndexMap {
+    const int32_t* idx;
+    int32_t base;
+    __device__ __forceinline__ int operator()(int t) const { return idx ? idx[t] : base + t; }
+};
+
 struct DevState {
     double* xd;
     double* yd;

@@ -248,7 +248,7 @@ extern "C" lq_status lq_adami_boundary(const lq_params* par, const lq_state* st,
         return LQ_ERR_INVALID_ARG;
     }
     if (par->npb == 0) return LQ_OK;  // src/adami_boundary.cu:135
-    launch_adami_one(one_desc(par, nullptr), to_dev(*st), cell_beg, cell_end, par->np, (hipStream_t)stream);
+    launch_adami_one(one_desc(par, nullptr), to_dev(*st), cell_beg, cell_end, (hipStream_t)stream);
     return last_launch("lq_adami_boundary");
 }
 
@@ -261,8 +261,8 @@ extern "C" lq_status lq_particle_interaction(const lq_params* par, const lq_stat
         return LQ_ERR_INVALID_ARG;
     }
     if (par->npm == 0) return LQ_OK;
-    launch_interaction_one(one_desc(par, nullptr), to_dev(*st), to_dev(*rates), cell_beg, cell_end, par->np,
-                           nbr_count, g_variant, (hipStream_t)stream);
+    launch_interaction_one(one_desc(par, nullptr), to_dev(*st), to_dev(*rates), cell_beg, cell_end, nbr_count,
+                           g_variant, (hipStream_t)stream);
     return last_launch("lq_particle_interaction");
 }
 
@@ -278,8 +278,7 @@ static lq_status integrate(bool corr, const lq_params* par, const lq_vw_params* 
         return LQ_ERR_INVALID_ARG;
     }
     if (par->npm == 0) return LQ_OK;  // src/predictor_corrector.cu:238,327
-    launch_integrate_one(corr, one_desc(par, vw), to_dev(*st), to_dev(*ini), to_dev(*rates), par->np,
-                         (hipStream_t)stream);
+    launch_integrate_one(corr, one_desc(par, vw), to_dev(*st), to_dev(*ini), to_dev(*rates), (hipStream_t)stream);
     return last_launch(corr ? "lq_corrector" : "lq_predictor");
 }
 
@@ -325,6 +324,8 @@ struct Buffers {
     int32_t* run_of;
     RunDesc* desc;
     int32_t* fail;  // per-run flag, raised on the device by the corrector
+    int32_t* mat_idx;   // particle index of the t-th material particle (all runs)
+    int32_t* wall_idx;  // particle index of the t-th wall particle (all runs)
     GridBuffers grid;
 };
 
@@ -348,6 +349,7 @@ struct lq_engine {
     std::vector<RunSlot> runs;  // layout order
     int cur = 0;
     int64_t np_total = 0, ncell_total = 0;
+    int64_t nmat_total = 0, nwall_total = 0;
     int32_t next_id = 0;
     bool use_graphs = true;
     hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -416,18 +418,20 @@ void enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
             launch_permute(b.grid.vals[b.grid.sorted_slot], n, b.S[c], b.S[1 - c], nullptr, nullptr, nullptr, s);
         }
         c = 1 - c;
+        const IndexMap walls{b.wall_idx, 0}, mats{b.mat_idx, 0};
+        const int nw = (int)e->nwall_total, nm = (int)e->nmat_total;
         {
             StageTimer t(prof, s, PG_ADAMI);
-            launch_adami_table(b.desc, b.run_of, b.S[c], b.grid.cell_beg, b.grid.cell_end, n, s);
+            launch_adami_table(b.desc, b.run_of, b.S[c], b.grid.cell_beg, b.grid.cell_end, walls, nw, s);
         }
         {
             StageTimer t(prof, s, PG_INTER);
-            launch_interaction_table(b.desc, b.run_of, b.S[c], b.R, b.grid.cell_beg, b.grid.cell_end, n, nullptr,
-                                     var, s);
+            launch_interaction_table(b.desc, b.run_of, b.S[c], b.R, b.grid.cell_beg, b.grid.cell_end, mats, nm,
+                                     nullptr, var, s);
         }
         {
             StageTimer t(prof, s, PG_INTEG);
-            launch_integrate_table(corr, b.desc, b.run_of, b.S[c], b.I, b.R, n,
+            launch_integrate_table(corr, b.desc, b.run_of, b.S[c], b.I, b.R, mats, nm,
                                    corr ? b.grid.vals[b.grid.sorted_slot] : nullptr, b.fail, s);
         }
     }
@@ -464,17 +468,29 @@ void host_step_end(lq_engine* e) {
     }
 }
 
+// Uploads the run table, the failure flags and the wall / material index maps
+// of the current layout.
 lq_status upload_descs(lq_engine* e) {
     std::vector<RunDesc> h(e->runs.size());
     std::vector<int32_t> f(e->runs.size());
+    std::vector<int32_t> walls, mats;
     for (size_t i = 0; i < e->runs.size(); ++i) {
-        h[i] = e->runs[i].desc;
+        const RunDesc& d = e->runs[i].desc;
+        h[i] = d;
         f[i] = e->runs[i].failed;
+        for (int k = 0; k < d.npb; ++k) walls.push_back(d.p_off + k);
+        for (int k = d.npb; k < d.np; ++k) mats.push_back(d.p_off + k);
     }
+    e->nwall_total = (int64_t)walls.size();
+    e->nmat_total = (int64_t)mats.size();
     if (!h.empty()) {
         LQ_HIP_TRY(hipMemcpyAsync(e->b.desc, h.data(), h.size() * sizeof(RunDesc), hipMemcpyHostToDevice, e->stream));
         LQ_HIP_TRY(hipMemcpyAsync(e->b.fail, f.data(), f.size() * 4, hipMemcpyHostToDevice, e->stream));
     }
+    if (!walls.empty())
+        LQ_HIP_TRY(hipMemcpyAsync(e->b.wall_idx, walls.data(), walls.size() * 4, hipMemcpyHostToDevice, e->stream));
+    if (!mats.empty())
+        LQ_HIP_TRY(hipMemcpyAsync(e->b.mat_idx, mats.data(), mats.size() * 4, hipMemcpyHostToDevice, e->stream));
     LQ_HIP_TRY(hipStreamSynchronize(e->stream));
     return LQ_OK;
 }
@@ -527,8 +543,9 @@ lq_status relayout(lq_engine* e) {
 
 extern "C" int64_t lq_engine_bytes_per_run(int64_t np, int64_t ncell) {
     // 2 states (124 B) + ini (96 B) + rates (56 B) + Shepard scratch (24 B)
-    // + run id (4 B) + sort keys/values (16 B) + histograms, + table 8 B/cell.
-    const int64_t per_particle = 2 * 124 + 96 + 56 + 24 + 4 + 16 + 1;
+    // + run id (4 B) + wall/material index (4 B) + sort keys/values (16 B)
+    // + histograms, + table 8 B/cell.
+    const int64_t per_particle = 2 * 124 + 96 + 56 + 24 + 4 + 4 + 16 + 1;
     return np * per_particle + ncell * 8 + (int64_t)sizeof(RunDesc);
 }
 
@@ -565,7 +582,8 @@ extern "C" lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t
     ok = ok && dmalloc(b.R.accad, np_cap, e->owned) && dmalloc(b.R.spin, np_cap, e->owned) &&
          dmalloc(b.R.acestn, np_cap, e->owned) && dmalloc(b.T, np_cap, e->owned) &&
          dmalloc(b.run_of, np_cap, e->owned) && dmalloc(b.desc, max_runs, e->owned) &&
-         dmalloc(b.fail, max_runs, e->owned);
+         dmalloc(b.fail, max_runs, e->owned) && dmalloc(b.mat_idx, np_cap, e->owned) &&
+         dmalloc(b.wall_idx, np_cap, e->owned);
     if (!ok || alloc_grid(b.grid, np_cap, ncell_cap) != LQ_OK) {
         for (void* p : e->owned) (void)hipFree(p);
         (void)hipStreamDestroy(e->stream);

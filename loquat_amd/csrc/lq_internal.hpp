@@ -76,23 +76,25 @@ void launch_permute(const uint32_t* perm, int n, const DevState& src, const DevS
 // ---------------------------------------------------------------- physics
 enum Stage { ST_ADAMI = 0, ST_INTERACT = 1, ST_PREDICT = 2, ST_CORRECT = 3, ST_SHEPARD = 4 };
 
-void launch_adami_one(const RunDesc& d, const DevState& st, const int32_t* beg, const int32_t* end, int n,
-                      hipStream_t s);
+// _one: a single run by value (walls [0, npb), material [npb, np)).
+// _table: all resident runs; walls / mats map a compact thread index to the
+// particle index of the t-th wall / material particle.
+void launch_adami_one(const RunDesc& d, const DevState& st, const int32_t* beg, const int32_t* end, hipStream_t s);
 void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const int32_t* beg,
-                        const int32_t* end, int n, hipStream_t s);
+                        const int32_t* end, IndexMap walls, int nwalls, hipStream_t s);
 void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates& rt, const int32_t* beg,
-                            const int32_t* end, int n, int32_t* nbr, int variant, hipStream_t s);
+                            const int32_t* end, int32_t* nbr, int variant, hipStream_t s);
 void launch_interaction_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt,
-                              const int32_t* beg, const int32_t* end, int n, int32_t* nbr, int variant,
-                              hipStream_t s);
+                              const int32_t* beg, const int32_t* end, IndexMap mats, int nmat, int32_t* nbr,
+                              int variant, hipStream_t s);
 void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, const DevIni& ini,
-                          const DevRates& rt, int n, hipStream_t s);
+                          const DevRates& rt, hipStream_t s);
 // ini_perm: the corrector reads particle k's ini entry at ini_perm[k] (the
 // last sort's permutation; nullable = identity).  fail: per-run flags raised
 // by the corrector on a non-finite state (nullable).
 void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
-                            const DevIni& ini, const DevRates& rt, int n, const uint32_t* ini_perm, int32_t* fail,
-                            hipStream_t s);
+                            const DevIni& ini, const DevRates& rt, IndexMap mats, int nmat, const uint32_t* ini_perm,
+                            int32_t* fail, hipStream_t s);
 // Shepard filter: reads stress from str_src, writes str_dst (walls and runs
 // not due are copied through).
 void launch_shepard_one(const RunDesc& d, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,

@@ -24,6 +24,19 @@ namespace {
 
 inline int blocks_for(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
+// Block index remapped so that each XCD works on one contiguous slice of the
+// launch's index space.  Workgroups are dealt to the 8 XCDs round-robin;
+// without the remap every XCD touches the whole domain and spatial
+// neighbours are fetched into all eight L2s.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int xcd = b & 7, idx = b >> 3;
+    const int q = nb >> 3, r = nb & 7;
+    return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+// Compact thread index of this thread in an XCD-contiguous launch.
+__device__ __forceinline__ int xcd_thread() { return xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; }
+
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
@@ -52,11 +65,13 @@ __device__ __forceinline__ tensor2d jaumann(const float4 w, const tensor2d& s) {
 }
 
 // ------------------------------------------------------------ Adami walls
+// n threads over the walls of all runs, walls(t) = particle index.
 template <class DP>
 __global__ __launch_bounds__(256) void k_adami(DP dp, DevState st, const int32_t* __restrict__ beg,
-                                               const int32_t* __restrict__ end, int n) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+                                               const int32_t* __restrict__ end, IndexMap walls, int n) {
+    const int t = xcd_thread();
+    if (t >= n) return;
+    const int k = walls(t);
     const RunDesc& d = dp.get(dp.run(k));
     if ((k - d.p_off) >= d.npb || !d.active) return;
     const float4 pi = st.pospres[k];
@@ -199,14 +214,17 @@ __device__ __forceinline__ void store_rates(const RunDesc& d, const DevRates& rt
     rt.acestn[k] = e;
 }
 
+// All interaction variants run n threads over the material particles of all
+// runs, mats(t) = particle index.
 // Variant 0: each thread walks its own candidates and evaluates hits inline.
 template <class DP>
 __global__ __launch_bounds__(256) void k_interaction_direct(DP dp, DevState st, DevRates rt,
                                                             const int32_t* __restrict__ beg,
-                                                            const int32_t* __restrict__ end, int n,
-                                                            int32_t* __restrict__ nbr) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+                                                            const int32_t* __restrict__ end, IndexMap mats,
+                                                            int n, int32_t* __restrict__ nbr) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int k = mats(t);
     const RunDesc& d = dp.get(dp.run(k));
     if ((k - d.p_off) < d.npb || !d.active) return;
     const IConst c = make_iconst(d);
@@ -241,15 +259,16 @@ constexpr int IC_CAP = 32;
 template <class DP>
 __global__ __launch_bounds__(256) void k_interaction_lds(DP dp, DevState st, DevRates rt,
                                                          const int32_t* __restrict__ beg,
-                                                         const int32_t* __restrict__ end, int n,
-                                                         int32_t* __restrict__ nbr) {
+                                                         const int32_t* __restrict__ end, IndexMap mats,
+                                                         int n, int32_t* __restrict__ nbr) {
     __shared__ int32_t lists[4][IC_CAP][64];
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = t < n ? mats(t) : -1;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int32_t(*my)[64] = lists[w];
     bool mat = false;
     const RunDesc* dptr = nullptr;
-    if (k < n) {
+    if (t < n) {
         const RunDesc& d = dp.get(dp.run(k));
         dptr = &d;
         mat = (k - d.p_off) >= d.npb && d.active;
@@ -340,15 +359,16 @@ constexpr int IC2_U = 4;
 template <class DP>
 __global__ __launch_bounds__(256) void k_interaction_unrolled(DP dp, DevState st, DevRates rt,
                                                               const int32_t* __restrict__ beg,
-                                                              const int32_t* __restrict__ end, int n,
-                                                              int32_t* __restrict__ nbr) {
+                                                              const int32_t* __restrict__ end, IndexMap mats,
+                                                              int n, int32_t* __restrict__ nbr) {
     __shared__ int32_t lists[4][IC2_CAP][64];
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = t < n ? mats(t) : -1;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int32_t(*my)[64] = lists[w];
     bool mat = false;
     const RunDesc* dptr = nullptr;
-    if (k < n) {
+    if (t < n) {
         const RunDesc& d = dp.get(dp.run(k));
         dptr = &d;
         mat = (k - d.p_off) >= d.npb && d.active;
@@ -441,16 +461,6 @@ __global__ __launch_bounds__(256) void k_interaction_unrolled(DP dp, DevState st
     }
 }
 
-// Block index remapped so that each XCD works on one contiguous slice of the
-// sorted particles.  Workgroups are dealt to the 8 XCDs round-robin; without
-// the remap every XCD touches the whole domain and spatial neighbours are
-// fetched into all eight L2s.
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-    const int xcd = b & 7, idx = b >> 3;
-    const int q = nb >> 3, r = nb & 7;
-    return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
-}
-
 // 32-bit byte offset loads from a uniform base (saddr addressing: one shift
 // per load instead of 64-bit address arithmetic).
 template <class T>
@@ -508,15 +518,16 @@ __device__ __forceinline__ void pair_eval3(const IPart& pi, uint32_t j, const De
 template <class DP>
 __global__ __launch_bounds__(256) void k_interaction_v3(DP dp, DevState st, DevRates rt,
                                                         const int32_t* __restrict__ beg,
-                                                        const int32_t* __restrict__ end, int n,
+                                                        const int32_t* __restrict__ end, IndexMap mats, int n,
                                                         int32_t* __restrict__ nbr) {
     __shared__ int32_t lists[4][IC2_CAP][64];
-    const int k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int t = xcd_thread();
+    const int k = t < n ? mats(t) : -1;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int32_t(*my)[64] = lists[w];
     bool mat = false;
     const RunDesc* dptr = nullptr;
-    if (k < n) {
+    if (t < n) {
         const RunDesc& d = dp.get(dp.run(k));
         dptr = &d;
         mat = (k - d.p_off) >= d.npb && d.active;
@@ -772,9 +783,11 @@ __device__ __forceinline__ void constitutive(const RunDesc& d, float dt, const f
 // Predictor (src/predictor_corrector.cu:169-218) fused with the *_ini copy
 // that the reference does as five device-to-device copies (:240-246).
 template <class DP>
-__global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni ini, DevRates rt, int n) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+__global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni ini, DevRates rt, IndexMap mats,
+                                                   int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int k = mats(t);
     const RunDesc& d = dp.get(dp.run(k));
     if ((k - d.p_off) < d.npb || !d.active) return;
     const float dt = d.par.dt;
@@ -828,11 +841,12 @@ __global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni in
 // fail (nullable, one flag per run) is raised when a particle's new state is
 // not finite, so a diverging run can be stopped without touching the others.
 template <class DP>
-__global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni ini, DevRates rt, int n,
-                                                   const uint32_t* __restrict__ ini_perm,
+__global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni ini, DevRates rt, IndexMap mats,
+                                                   int n, const uint32_t* __restrict__ ini_perm,
                                                    int32_t* __restrict__ fail) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int k = mats(t);
     const int run = dp.run(k);
     const RunDesc& d = dp.get(run);
     if ((k - d.p_off) < d.npb || !d.active) return;
@@ -950,56 +964,61 @@ __global__ void k_copy_tensor(const tensor2d* __restrict__ src, tensor2d* __rest
 
 }  // namespace
 
-void launch_adami_one(const RunDesc& d, const DevState& st, const int32_t* beg, const int32_t* end, int n,
-                      hipStream_t s) {
-    if (n > 0) k_adami<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, beg, end, n);
+// One run: walls are [0, npb), material particles [npb, np).
+void launch_adami_one(const RunDesc& d, const DevState& st, const int32_t* beg, const int32_t* end, hipStream_t s) {
+    const int n = d.npb;
+    if (n > 0) k_adami<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, beg, end, IndexMap{nullptr, 0}, n);
 }
 void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const int32_t* beg,
-                        const int32_t* end, int n, hipStream_t s) {
-    if (n > 0) k_adami<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, st, beg, end, n);
+                        const int32_t* end, IndexMap walls, int nwalls, hipStream_t s) {
+    if (nwalls > 0)
+        k_adami<DescTable><<<blocks_for(nwalls, 256), 256, 0, s>>>(DescTable{d, run_of}, st, beg, end, walls, nwalls);
 }
 
 template <class DP>
 static void launch_interaction(const DP& dp, const DevState& st, const DevRates& rt, const int32_t* beg,
-                               const int32_t* end, int n, int32_t* nbr, int variant, hipStream_t s) {
+                               const int32_t* end, IndexMap mats, int n, int32_t* nbr, int variant, hipStream_t s) {
     if (n <= 0) return;
+    const int nb = blocks_for(n, 256);
     if (variant == 0)
-        k_interaction_direct<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, rt, beg, end, n, nbr);
+        k_interaction_direct<DP><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else if (variant == 1)
-        k_interaction_lds<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, rt, beg, end, n, nbr);
+        k_interaction_lds<DP><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else if (variant == 2)
-        k_interaction_unrolled<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, rt, beg, end, n, nbr);
+        k_interaction_unrolled<DP><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else
-        k_interaction_v3<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, rt, beg, end, n, nbr);
+        k_interaction_v3<DP><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
 }
 
 void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates& rt, const int32_t* beg,
-                            const int32_t* end, int n, int32_t* nbr, int variant, hipStream_t s) {
-    launch_interaction(DescOne{d}, st, rt, beg, end, n, nbr, variant, s);
+                            const int32_t* end, int32_t* nbr, int variant, hipStream_t s) {
+    launch_interaction(DescOne{d}, st, rt, beg, end, IndexMap{nullptr, d.npb}, d.np - d.npb, nbr, variant, s);
 }
 void launch_interaction_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt,
-                              const int32_t* beg, const int32_t* end, int n, int32_t* nbr, int variant,
-                              hipStream_t s) {
-    launch_interaction(DescTable{d, run_of}, st, rt, beg, end, n, nbr, variant, s);
+                              const int32_t* beg, const int32_t* end, IndexMap mats, int nmat, int32_t* nbr,
+                              int variant, hipStream_t s) {
+    launch_interaction(DescTable{d, run_of}, st, rt, beg, end, mats, nmat, nbr, variant, s);
 }
 
 void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, const DevIni& ini,
-                          const DevRates& rt, int n, hipStream_t s) {
+                          const DevRates& rt, hipStream_t s) {
+    const int n = d.np - d.npb;
     if (n <= 0) return;
+    const IndexMap mats{nullptr, d.npb};
     if (corrector)
-        k_corrector<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, n, nullptr, nullptr);
+        k_corrector<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, mats, n, nullptr, nullptr);
     else
-        k_predictor<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, n);
+        k_predictor<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, mats, n);
 }
 void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
-                            const DevIni& ini, const DevRates& rt, int n, const uint32_t* ini_perm, int32_t* fail,
-                            hipStream_t s) {
-    if (n <= 0) return;
+                            const DevIni& ini, const DevRates& rt, IndexMap mats, int nmat, const uint32_t* ini_perm,
+                            int32_t* fail, hipStream_t s) {
+    if (nmat <= 0) return;
+    const int nb = blocks_for(nmat, 256);
     if (corrector)
-        k_corrector<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, n, ini_perm,
-                                                                   fail);
+        k_corrector<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat, ini_perm, fail);
     else
-        k_predictor<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, n);
+        k_predictor<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat);
 }
 
 void launch_shepard_one(const RunDesc& d, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,
