@@ -272,7 +272,9 @@ def main() -> None:
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variant", type=int, default=3, help="interaction kernel variant")
+    ap.add_argument("--variant", type=int, default=3, help="interaction kernel variant (without --autotune)")
+    ap.add_argument("--autotune", type=int, default=1, help="1: time every interaction variant on the "
+                    "workload's state and run the fastest")
     ap.add_argument("--ensemble-runs", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--serve-requests", type=int, default=48)
@@ -305,6 +307,12 @@ def main() -> None:
 
     eng = Engine(case.np, p.ncell, 1)
     rid = eng.add(case)
+    tuned = None
+    if a.autotune:
+        best, ms = eng.autotune(3)
+        tuned = {"variant": best, "kernel": KERNEL_NAMES[best], "launch_ms": [round(m, 4) for m in ms]}
+        log(f"autotune: variant {best}, launch ms per variant {tuned['launch_ms']}")
+    variant = tuned["variant"] if tuned else a.variant
     eng.step(a.warmup)
     eng.sync()
     torch.cuda.synchronize()
@@ -363,7 +371,7 @@ def main() -> None:
             "steps_done": info["steps"],
         },
         "roofline": {
-            "kernel": KERNEL_NAMES[a.variant],
+            "kernel": KERNEL_NAMES[variant],
             "bound": "hbm",
             "achieved": round(gbs, 2),
             "peak": HBM_PEAK_GBS,
@@ -382,6 +390,7 @@ def main() -> None:
         },
         "stage_ms_per_step": {k: round(v, 4) for k, v in
                               zip(("grid", "adami", "interaction", "integrate", "shepard"), stage)},
+        "autotune": tuned,
         "cpu_baseline": None,
     }
     torch.cuda.empty_cache()

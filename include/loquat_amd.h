@@ -277,6 +277,12 @@ lq_status lq_engine_set_graphs(lq_engine* eng, int32_t use_graphs);
  * (no graphs). */
 lq_status lq_engine_profile(lq_engine* eng, int32_t nsteps, double* ms_stage5);
 
+/* Autotuner: times nrep launches of each interaction variant on the engine's
+ * resident state and pins the fastest for this engine (all variants produce
+ * the same neighbour sets and sum in the same order).  ms_variant4
+ * (nullable) receives the mean launch time of variants 0..3. */
+lq_status lq_engine_autotune(lq_engine* eng, int32_t nrep, int32_t* best, double* ms_variant4);
+
 /* Interaction kernel variant for the engine and lq_particle_interaction:
  * 0 = per-thread direct loop, 1 = LDS-compacted neighbour lists,
  * 2 = LDS lists with a 4-wide unrolled candidate scan,

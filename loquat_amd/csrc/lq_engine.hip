@@ -352,6 +352,7 @@ struct lq_engine {
     int64_t nmat_total = 0, nwall_total = 0;
     int32_t next_id = 0;
     bool use_graphs = true;
+    int variant = -1;  // interaction variant pinned by lq_engine_autotune (-1: library default)
     hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 };
 
@@ -405,7 +406,7 @@ void enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
     hipStream_t s = e->stream;
     const int n = (int)e->np_total;
     const int nr = (int)e->runs.size();
-    const int var = g_variant;
+    const int var = e->variant >= 0 ? e->variant : g_variant;
     launch_step_begin(b.desc, nr, s);
     int c = cur;
     for (int half = 0; half < 2; ++half) {
@@ -777,6 +778,56 @@ extern "C" lq_status lq_engine_step(lq_engine* e, int32_t nsteps) {
         if (shep) e->cur = 1 - e->cur;
         host_step_end(e);
     }
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_engine_autotune(lq_engine* e, int32_t nrep, int32_t* best, double* ms_variant4) {
+    if (!e || nrep <= 0) {
+        set_error("lq_engine_autotune: invalid argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (e->runs.empty() || e->nmat_total == 0) {
+        set_error("lq_engine_autotune: no material particles resident");
+        return LQ_ERR_STATE;
+    }
+    Buffers& b = e->b;
+    hipStream_t s = e->stream;
+    // Sort the resident state as a step would (the next step's stable sort by
+    // the same keys leaves it unchanged), so the neighbour table is current.
+    enqueue_build(e, b.S[e->cur], s);
+    launch_permute(b.grid.vals[b.grid.sorted_slot], (int)e->np_total, b.S[e->cur], b.S[1 - e->cur], nullptr, nullptr,
+                   nullptr, s);
+    e->cur = 1 - e->cur;
+    drop_graphs(e);
+    hipEvent_t ev[2];
+    LQ_HIP_TRY(hipEventCreate(&ev[0]));
+    LQ_HIP_TRY(hipEventCreate(&ev[1]));
+    const IndexMap mats{b.mat_idx, 0};
+    double best_ms = 1e30;
+    int best_v = 3;
+    for (int v = 0; v < 4; ++v) {
+        launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
+                                 (int)e->nmat_total, nullptr, v, s);  // warm-up
+        LQ_HIP_TRY(hipEventRecord(ev[0], s));
+        for (int r = 0; r < nrep; ++r)
+            launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
+                                     (int)e->nmat_total, nullptr, v, s);
+        LQ_HIP_TRY(hipEventRecord(ev[1], s));
+        LQ_HIP_TRY(hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        LQ_HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        const double per = ms / nrep;
+        if (ms_variant4) ms_variant4[v] = per;
+        if (per < best_ms) {
+            best_ms = per;
+            best_v = v;
+        }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    LQ_TRY(last_launch("lq_engine_autotune"));
+    e->variant = best_v;
+    if (best) *best = best_v;
     return LQ_OK;
 }
 

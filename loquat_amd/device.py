@@ -239,6 +239,13 @@ class Engine:
         out["posd"] = posd
         return out
 
+    def autotune(self, nrep: int = 3) -> tuple[int, list[float]]:
+        """Pins the fastest interaction variant for this engine."""
+        best = C.c_int32()
+        ms = (C.c_double * 4)()
+        abi.check(abi.lib().lq_engine_autotune(self.h, int(nrep), C.byref(best), ms))
+        return best.value, list(ms)
+
     def profile(self, nsteps: int) -> list[float]:
         ms = (C.c_double * 5)()
         abi.check(abi.lib().lq_engine_profile(self.h, int(nsteps), ms))

@@ -126,6 +126,23 @@ def test_interaction_variants_agree_in_engine(gpu):
         assert rel_err(res[0][k], res[1][k]) < 1e-5
 
 
+@pytest.mark.gpu
+def test_autotune_picks_a_variant_and_keeps_results(gpu):
+    c = cases.perturb(cases.small_column_3d(seed=51), seed=52)
+    e, (rid,) = _engine([c], 6)
+    best, ms = e.autotune(2)
+    assert best in (0, 1, 2, 3) and all(m > 0 for m in ms) and ms[best] == min(ms)
+    e.step(6)
+    tuned = e.download(rid)
+    e.close()
+    e, (rid,) = _engine([c], 6)
+    e.step(6)
+    plain = e.download(rid)
+    e.close()
+    for k in ("posd", "velrhop", "str"):
+        assert rel_err(tuned[k], plain[k]) < 1e-5, k
+
+
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c1_slab_trajectory.json")
 
 
