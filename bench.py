@@ -32,7 +32,8 @@ BYTES_READ_PER_PARTICLE = 64   # x (FP64, 24) + v (12) + rho (4) + stress (24)
 BYTES_WRITE_PER_MATERIAL = 52  # a (12) + drho (4) + spin (12) + strain rate (24)
 BYTES_PER_CELL = 8             # cell_beg + cell_end
 KERNEL_NAMES = {0: "k_interaction_direct", 1: "k_interaction_lds", 2: "k_interaction_unrolled",
-                3: "k_interaction_v3"}
+                3: "k_interaction_v3<DescTable, 1, false, 1, 4>", 4: "k_interaction_v3<DescTable, 2, false, 1, 4>",
+                5: "k_interaction_v3<DescTable, 1, true, 1, 4>", 6: "k_interaction_v3<DescTable, 2, true, 1, 4>"}
 
 
 def log(msg: str) -> None:
@@ -281,6 +282,8 @@ def main() -> None:
     ap.add_argument("--serve-step-scale", type=float, default=0.05)
     ap.add_argument("--slope-steps", type=int, default=20)
     ap.add_argument("--no-extra", action="store_true", help="skip the ensemble / served / slope sections")
+    ap.add_argument("--cell-order", type=int, default=4, metavar="B",
+                    help="in-cell order of the step's sort: 0 = by cell only, B > 0 = 2^B z-bins per cell")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="after the run, 5 device copies of MB MiB (reference kernel for HBM byte counters)")
     a = ap.parse_args()
@@ -306,6 +309,7 @@ def main() -> None:
     log(f"neighbour pairs of the initial state: {pairs} ({pairs / case.npm:.2f} per material particle)")
 
     eng = Engine(case.np, p.ncell, 1)
+    eng.set_cell_order(a.cell_order)
     rid = eng.add(case)
     tuned = None
     if a.autotune:
@@ -366,6 +370,7 @@ def main() -> None:
                         "phi=30 deg, Shepard every 20 steps, FP32 sums + FP64 positions",
             "npm": case.npm, "np": case.np, "ncell": p.ncell, "dx": 0.005,
             "particle_step": "one material particle advanced one full predictor-corrector step",
+            "cell_order_zbits": a.cell_order,
             "np_steps_per_s_M": round(case.np * a.steps / (gpu_ms * 1e-3) / 1e6, 3),
             "wall_ms_per_step": round(wall * 1e3 / a.steps, 4),
             "steps_done": info["steps"],

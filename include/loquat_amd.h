@@ -270,6 +270,21 @@ int64_t lq_engine_bytes_per_run(int64_t np, int64_t ncell);
 lq_stream lq_engine_stream(const lq_engine* eng);
 /* use_graphs: 1 = replay each step from a captured hipGraph (default), 0 = direct launches */
 lq_status lq_engine_set_graphs(lq_engine* eng, int32_t use_graphs);
+/* Order of particles inside a cell after each step's sort: zbin_bits = 0
+ * keeps the reference's stable order (by cell only, SortParticleByCellIndex,
+ * src/loquat_grid.h:43);
+ * b > 0 (default 4) sorts each cell by 2^b height bins, so that every stencil
+ * column is ordered by z and the neighbour scans read only the z-window of
+ * the kernel support.  Cell tables and neighbour sets are the same either way;
+ * only the order of the FP32 pair sums changes.  No reference counterpart. */
+lq_status lq_engine_set_cell_order(lq_engine* eng, int32_t zbin_bits);
+
+/* Neighbour count of every particle of a run, in the run's input order:
+ * material particles get the number of particles (walls and material, not
+ * itself) the interaction pairs them with on the resident state, walls 0.
+ * Sorts the resident state first (as a step does).  counts: host, np ints.
+ * Diagnostic; the count semantics are lq_particle_interaction's. */
+lq_status lq_engine_neighbour_counts(lq_engine* eng, int32_t run_id, int32_t* counts);
 
 /* Mean device time per step of each stage over the last lq_engine_profile
  * call: grid (hash+sort+table+permute), adami, interaction, integrate,
@@ -279,14 +294,21 @@ lq_status lq_engine_profile(lq_engine* eng, int32_t nsteps, double* ms_stage5);
 
 /* Autotuner: times nrep launches of each interaction variant on the engine's
  * resident state and pins the fastest for this engine (all variants produce
- * the same neighbour sets and sum in the same order).  ms_variant4
- * (nullable) receives the mean launch time of variants 0..3. */
-lq_status lq_engine_autotune(lq_engine* eng, int32_t nrep, int32_t* best, double* ms_variant4);
+ * the same neighbour sets and sum in the same order).  ms_per_variant
+ * (nullable, LQ_INTERACTION_VARIANTS entries) receives the mean launch time
+ * of each variant. */
+#define LQ_INTERACTION_VARIANTS 7
+lq_status lq_engine_autotune(lq_engine* eng, int32_t nrep, int32_t* best, double* ms_per_variant);
 
 /* Interaction kernel variant for the engine and lq_particle_interaction:
  * 0 = per-thread direct loop, 1 = LDS-compacted neighbour lists,
  * 2 = LDS lists with a 4-wide unrolled candidate scan,
- * 3 = variant 2 with offset loads, XCD-contiguous blocks (default). */
+ * 3 = variant 2 with offset loads, pruned column scans, XCD-contiguous
+ *     blocks (default),
+ * 4 = variant 3 evaluating the lists two pairs at a time,
+ * 5, 6 = variants 3, 4 where a lane past the end of its list loads nothing
+ *        (exec-masked gathers) instead of re-reading its own particle.
+ * Variants 3 and up also select the list-based Adami and Shepard scans. */
 lq_status lq_set_interaction_variant(int32_t variant);
 int32_t lq_get_interaction_variant(void);
 

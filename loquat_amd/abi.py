@@ -86,6 +86,7 @@ class LqServerConfig(C.Structure):
 
 
 REQ_STATE = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "rejected"}
+INTERACTION_VARIANTS = 7  # LQ_INTERACTION_VARIANTS
 
 
 class LqError(RuntimeError):
@@ -133,6 +134,8 @@ _PROTOS = {
     "lq_engine_bytes_per_run": (C.c_int64, [C.c_int64, C.c_int64]),
     "lq_engine_stream": (vp, [vp]),
     "lq_engine_set_graphs": (i32, [vp, i32]),
+    "lq_engine_set_cell_order": (i32, [vp, i32]),
+    "lq_engine_neighbour_counts": (i32, [vp, i32, vp]),
     "lq_engine_profile": (i32, [vp, i32, C.POINTER(f64)]),
     "lq_engine_poll": (i32, [vp, C.POINTER(i32)]),
     "lq_engine_autotune": (i32, [vp, i32, C.POINTER(i32), C.POINTER(f64)]),

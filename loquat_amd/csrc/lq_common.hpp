@@ -33,13 +33,20 @@ struct RunDesc {
     int32_t shepard_due;  // this step applies the stress Shepard filter
     int32_t step;         // steps completed
     int32_t nsteps;       // steps requested
+    int32_t zbits;        // sort key = cell << zbits | z-bin in cell (0: cell only)
     double time;          // FP64 simulated time (src/loquat.cu:305,455)
     lq_vw_params vw;      // constitutive model 3 constants
     float vw_a, vw_fb0;   // derived model-3 constants (a, and the f_b prefactor)
 };
 
 // Descriptor access for one run passed by value (module-level calls).
+// `padded`: the particle arrays carry SCAN_PAD spare entries past the last
+// particle, so a neighbour scan may read up to three candidates beyond the end
+// of a range without clamping (engine buffers; module calls get caller arrays).
+constexpr int SCAN_PAD = 4;
+
 struct DescOne {
+    static constexpr bool padded = false;
     RunDesc d;
     __device__ __forceinline__ int run(int) const { return 0; }
     __device__ __forceinline__ const RunDesc& get(int) const { return d; }
@@ -47,6 +54,7 @@ struct DescOne {
 
 // Descriptor access through a device table and a per-particle run id.
 struct DescTable {
+    static constexpr bool padded = true;
     const RunDesc* d;
     const int32_t* run_of;
     __device__ __forceinline__ int run(int k) const { return run_of[k]; }
@@ -113,6 +121,18 @@ __device__ __forceinline__ int cell_coord(float x, float i_h) {
     if (!(f >= -2.0f)) return -2;
     if (f > 1073741824.0f) return 1073741824;
     return (int)f;
+}
+
+// z-bin of a particle inside its cell: floor(s * 2^zbits) - cz * 2^zbits with
+// s = (0.5 z) / h the cell-unit coordinate and cz = cell_coord(z).  Scaling by
+// a power of two is exact, so the global bin (cz << zbits) + bin is
+// non-decreasing in z: sorting a cell by it sorts every column of cells by z
+// to within one bin (cell height / 2^zbits).
+__device__ __forceinline__ uint32_t zbin_in_cell(float z, float i_h, int zbits, int cz) {
+#pragma clang fp contract(off)
+    const float nb = (float)(1 << zbits);
+    const float f = floorf(((0.5f * z) * i_h) * nb) - (float)cz * nb;
+    return (uint32_t)fminf(fmaxf(f, 0.0f), nb - 1.0f);
 }
 
 // FP32 squared distance with a fixed operation order and no contraction; the

@@ -145,8 +145,8 @@ extern "C" lq_status lq_neighbour_r2_cut(const lq_params* par, float* r2_cut) {
 }
 
 extern "C" lq_status lq_set_interaction_variant(int32_t v) {
-    if (v < 0 || v > 3) {
-        set_error("interaction variant must be 0, 1, 2 or 3");
+    if (v < 0 || v >= LQ_INTERACTION_VARIANTS) {
+        set_error("interaction variant must be 0..6");
         return LQ_ERR_INVALID_ARG;
     }
     g_variant = v;
@@ -196,7 +196,7 @@ extern "C" lq_status lq_grid_build(lq_grid* grid, const lq_params* par, const fl
     const RunDesc d = one_desc(par, nullptr);
     launch_hash_one(d, (const float4*)pospres, par->np, grid->g.keys[0], s);
     launch_radix_sort(grid->g, par->np, rs_bits(par->ncell), s);
-    launch_cell_table(grid->g, par->np, par->ncell, s);
+    launch_cell_table(grid->g, par->np, par->ncell, 0, s);
     grid->n = par->np;
     return last_launch("lq_grid_build");
 }
@@ -248,7 +248,7 @@ extern "C" lq_status lq_adami_boundary(const lq_params* par, const lq_state* st,
         return LQ_ERR_INVALID_ARG;
     }
     if (par->npb == 0) return LQ_OK;  // src/adami_boundary.cu:135
-    launch_adami_one(one_desc(par, nullptr), to_dev(*st), cell_beg, cell_end, (hipStream_t)stream);
+    launch_adami_one(one_desc(par, nullptr), to_dev(*st), cell_beg, cell_end, g_variant, (hipStream_t)stream);
     return last_launch("lq_adami_boundary");
 }
 
@@ -301,7 +301,8 @@ extern "C" lq_status lq_field_smoothing(const lq_params* par, const lq_state* st
     if (par->npm == 0) return LQ_OK;  // src/field_smoothing.cu:134
     hipStream_t s = (hipStream_t)stream;
     DevState ds = to_dev(*st);
-    launch_shepard_one(one_desc(par, nullptr), ds, ds.str, (tensor2d*)str_cache, cell_beg, cell_end, par->np, s);
+    launch_shepard_one(one_desc(par, nullptr), ds, ds.str, (tensor2d*)str_cache, cell_beg, cell_end, par->np,
+                       g_variant, s);
     launch_copy_tensor((const tensor2d*)str_cache, ds.str, par->np, s);
     return last_launch("lq_field_smoothing");
 }
@@ -353,10 +354,14 @@ struct lq_engine {
     int32_t next_id = 0;
     bool use_graphs = true;
     int variant = -1;  // interaction variant pinned by lq_engine_autotune (-1: library default)
+    int zbits = 4;     // z-bins per cell (log2) in the engine's sort key
     hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 };
 
 namespace {
+
+// z-bin bits that fit beside the cell id in a 32-bit sort key.
+int zbits_eff(const lq_engine* e) { return std::max(0, std::min(e->zbits, 32 - rs_bits(e->ncell_total))); }
 
 void drop_graphs(lq_engine* e) {
     for (int i = 0; i < 2; ++i)
@@ -395,8 +400,9 @@ void enqueue_build(lq_engine* e, const DevState& st, hipStream_t s) {
     Buffers& b = e->b;
     const int n = (int)e->np_total;
     launch_hash_table(b.desc, b.run_of, st.pospres, n, b.grid.keys[0], s);
-    launch_radix_sort(b.grid, n, rs_bits(e->ncell_total), s);
-    launch_cell_table(b.grid, n, e->ncell_total, s);
+    const int zb = zbits_eff(e);
+    launch_radix_sort(b.grid, n, rs_bits(e->ncell_total) + zb, s);
+    launch_cell_table(b.grid, n, e->ncell_total, zb, s);
 }
 
 // One full step for every resident run, starting from buffer e->cur (the
@@ -423,7 +429,7 @@ void enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
         const int nw = (int)e->nwall_total, nm = (int)e->nmat_total;
         {
             StageTimer t(prof, s, PG_ADAMI);
-            launch_adami_table(b.desc, b.run_of, b.S[c], b.grid.cell_beg, b.grid.cell_end, walls, nw, s);
+            launch_adami_table(b.desc, b.run_of, b.S[c], b.grid.cell_beg, b.grid.cell_end, walls, nw, var, s);
         }
         {
             StageTimer t(prof, s, PG_INTER);
@@ -444,7 +450,8 @@ void enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
         }
         c = 1 - c;
         StageTimer t(prof, s, PG_SHEP);
-        launch_shepard_table(b.desc, b.run_of, b.S[c], b.T, b.S[c].str, b.grid.cell_beg, b.grid.cell_end, n, s);
+        launch_shepard_table(b.desc, b.run_of, b.S[c], b.T, b.S[c].str, b.grid.cell_beg, b.grid.cell_end, n, var,
+                             s);
     }
     launch_step_end(b.desc, nr, b.fail, s);
 }
@@ -475,8 +482,10 @@ lq_status upload_descs(lq_engine* e) {
     std::vector<RunDesc> h(e->runs.size());
     std::vector<int32_t> f(e->runs.size());
     std::vector<int32_t> walls, mats;
+    const int zb = zbits_eff(e);
     for (size_t i = 0; i < e->runs.size(); ++i) {
-        const RunDesc& d = e->runs[i].desc;
+        RunDesc& d = e->runs[i].desc;
+        d.zbits = zb;
         h[i] = d;
         f[i] = e->runs[i].failed;
         for (int k = 0; k < d.npb; ++k) walls.push_back(d.p_off + k);
@@ -569,7 +578,7 @@ extern "C" lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t
     for (int i = 0; i < 2 && ok; ++i) {
         DevState& S = b.S[i];
         ok = ok && dmalloc(S.xd, np_cap, e->owned) && dmalloc(S.yd, np_cap, e->owned) &&
-             dmalloc(S.zd, np_cap, e->owned) && dmalloc(S.pospres, np_cap, e->owned) &&
+             dmalloc(S.zd, np_cap, e->owned) && dmalloc(S.pospres, np_cap + SCAN_PAD, e->owned) &&
              dmalloc(S.velrhop, np_cap, e->owned) && dmalloc(S.idep, np_cap, e->owned) &&
              dmalloc(S.str, np_cap, e->owned) && dmalloc(S.stn, np_cap, e->owned) &&
              dmalloc(S.orig, np_cap, e->owned);
@@ -615,6 +624,18 @@ extern "C" lq_status lq_engine_set_graphs(lq_engine* e, int32_t use_graphs) {
         return LQ_ERR_INVALID_ARG;
     }
     e->use_graphs = use_graphs != 0;
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_engine_set_cell_order(lq_engine* e, int32_t zbin_bits) {
+    if (!e || zbin_bits < 0 || zbin_bits > 8) {
+        set_error("lq_engine_set_cell_order: zbin_bits must be 0..8");
+        return LQ_ERR_INVALID_ARG;
+    }
+    LQ_HIP_TRY(hipStreamSynchronize(e->stream));
+    e->zbits = zbin_bits;
+    LQ_TRY(upload_descs(e));
+    drop_graphs(e);
     return LQ_OK;
 }
 
@@ -781,7 +802,60 @@ extern "C" lq_status lq_engine_step(lq_engine* e, int32_t nsteps) {
     return LQ_OK;
 }
 
-extern "C" lq_status lq_engine_autotune(lq_engine* e, int32_t nrep, int32_t* best, double* ms_variant4) {
+// Sorts the resident state as a step would (the next step's stable sort by
+// the same keys leaves it unchanged), so that the neighbour table is current.
+static void sort_resident(lq_engine* e) {
+    Buffers& b = e->b;
+    enqueue_build(e, b.S[e->cur], e->stream);
+    launch_permute(b.grid.vals[b.grid.sorted_slot], (int)e->np_total, b.S[e->cur], b.S[1 - e->cur], nullptr, nullptr,
+                   nullptr, e->stream);
+    e->cur = 1 - e->cur;
+}
+
+extern "C" lq_status lq_engine_neighbour_counts(lq_engine* e, int32_t run_id, int32_t* counts) {
+    if (!e || !counts) {
+        set_error("lq_engine_neighbour_counts: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    const int i = find_run(e, run_id);
+    if (i < 0) {
+        set_error("unknown run id");
+        return LQ_ERR_INVALID_ARG;
+    }
+    Buffers& b = e->b;
+    hipStream_t s = e->stream;
+    LQ_HIP_TRY(hipStreamSynchronize(s));
+    sort_resident(e);
+    int32_t* nbr = nullptr;
+    LQ_HIP_TRY(hipMalloc(&nbr, (size_t)std::max<int64_t>(e->np_total, 1) * 4));
+    const IndexMap mats{b.mat_idx, 0};
+    const int var = e->variant >= 0 ? e->variant : g_variant;
+    launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
+                             (int)e->nmat_total, nbr, var, s);
+    const RunDesc& d = e->runs[i].desc;
+    std::vector<int32_t> c(d.np), orig(d.np);
+    hipError_t err = hipGetLastError();
+    if (err == hipSuccess && d.np > 0) err = hipMemcpyAsync(c.data(), nbr + d.p_off, d.np * 4, hipMemcpyDeviceToHost, s);
+    if (err == hipSuccess && d.np > 0)
+        err = hipMemcpyAsync(orig.data(), b.S[e->cur].orig + d.p_off, d.np * 4, hipMemcpyDeviceToHost, s);
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
+    (void)hipFree(nbr);
+    if (err != hipSuccess) {
+        set_error(std::string("lq_engine_neighbour_counts: ") + hipGetErrorString(err));
+        return LQ_ERR_HIP;
+    }
+    for (int k = 0; k < d.np; ++k) {
+        const int t = orig[k];
+        if (t < 0 || t >= d.np) {
+            set_error("corrupt original-index array");
+            return LQ_ERR_STATE;
+        }
+        counts[t] = k < d.npb ? 0 : c[k];
+    }
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_engine_autotune(lq_engine* e, int32_t nrep, int32_t* best, double* ms_per_variant) {
     if (!e || nrep <= 0) {
         set_error("lq_engine_autotune: invalid argument");
         return LQ_ERR_INVALID_ARG;
@@ -792,12 +866,7 @@ extern "C" lq_status lq_engine_autotune(lq_engine* e, int32_t nrep, int32_t* bes
     }
     Buffers& b = e->b;
     hipStream_t s = e->stream;
-    // Sort the resident state as a step would (the next step's stable sort by
-    // the same keys leaves it unchanged), so the neighbour table is current.
-    enqueue_build(e, b.S[e->cur], s);
-    launch_permute(b.grid.vals[b.grid.sorted_slot], (int)e->np_total, b.S[e->cur], b.S[1 - e->cur], nullptr, nullptr,
-                   nullptr, s);
-    e->cur = 1 - e->cur;
+    sort_resident(e);
     drop_graphs(e);
     hipEvent_t ev[2];
     LQ_HIP_TRY(hipEventCreate(&ev[0]));
@@ -805,7 +874,7 @@ extern "C" lq_status lq_engine_autotune(lq_engine* e, int32_t nrep, int32_t* bes
     const IndexMap mats{b.mat_idx, 0};
     double best_ms = 1e30;
     int best_v = 3;
-    for (int v = 0; v < 4; ++v) {
+    for (int v = 0; v < LQ_INTERACTION_VARIANTS; ++v) {
         launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
                                  (int)e->nmat_total, nullptr, v, s);  // warm-up
         LQ_HIP_TRY(hipEventRecord(ev[0], s));
@@ -817,7 +886,7 @@ extern "C" lq_status lq_engine_autotune(lq_engine* e, int32_t nrep, int32_t* bes
         float ms = 0.f;
         LQ_HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
         const double per = ms / nrep;
-        if (ms_variant4) ms_variant4[v] = per;
+        if (ms_per_variant) ms_per_variant[v] = per;
         if (per < best_ms) {
             best_ms = per;
             best_v = v;

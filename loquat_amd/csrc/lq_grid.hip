@@ -185,9 +185,10 @@ __global__ __launch_bounds__(256) void k_hash(DP dp, const float4* __restrict__ 
     const int nx = d.par.cell_num_x, ny = d.par.cell_num_y, nz = d.par.cell_num_z;
     uint32_t key;
     if (cx < 0 || cx >= nx || cy < 0 || cy >= ny || cz < 0 || cz >= nz)
-        key = (uint32_t)(d.cell_base + d.par.ncell - 1);
+        key = (uint32_t)(d.cell_base + d.par.ncell - 1) << d.zbits;
     else
-        key = (uint32_t)(d.cell_base + layer * d.par.ncell_one_layer + (cx * ny + cy) * nz + cz);
+        key = ((uint32_t)(d.cell_base + layer * d.par.ncell_one_layer + (cx * ny + cy) * nz + cz) << d.zbits) |
+              (d.zbits ? zbin_in_cell(p.z, i_h, d.zbits, cz) : 0u);
     keys[k] = key;
 }
 
@@ -198,13 +199,15 @@ __global__ void k_fill2(int32_t* __restrict__ a, int32_t* __restrict__ b, int64_
     }
 }
 
-__global__ void k_cell_table(const uint32_t* __restrict__ keys, int n, int32_t* __restrict__ beg,
+// Begin/end of every occupied cell; the cell is the key without its low
+// zbits (the z-bin inside the cell).
+__global__ void k_cell_table(const uint32_t* __restrict__ keys, int n, int zbits, int32_t* __restrict__ beg,
                              int32_t* __restrict__ end) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const uint32_t key = keys[k];
-    if (k == 0 || keys[k - 1] != key) beg[key] = k;
-    if (k == n - 1 || keys[k + 1] != key) end[key] = k + 1;
+    const uint32_t cell = keys[k] >> zbits;
+    if (k == 0 || (keys[k - 1] >> zbits) != cell) beg[cell] = k;
+    if (k == n - 1 || (keys[k + 1] >> zbits) != cell) end[cell] = k + 1;
 }
 
 __global__ __launch_bounds__(256) void k_permute(const uint32_t* __restrict__ perm, int n, DevState src, DevState dst,
@@ -311,10 +314,11 @@ void launch_radix_sort(GridBuffers& g, int n, int bits, hipStream_t s) {
     g.sorted_slot = src;
 }
 
-void launch_cell_table(GridBuffers& g, int n, int64_t ncell_total, hipStream_t s) {
+void launch_cell_table(GridBuffers& g, int n, int64_t ncell_total, int zbits, hipStream_t s) {
     const int fill_blocks = (int)std::min<int64_t>(blocks_for(ncell_total, 256), 4096);
     if (ncell_total > 0) k_fill2<<<fill_blocks, 256, 0, s>>>(g.cell_beg, g.cell_end, ncell_total, -1);
-    if (n > 0) k_cell_table<<<blocks_for(n, 256), 256, 0, s>>>(g.keys[g.sorted_slot], n, g.cell_beg, g.cell_end);
+    if (n > 0)
+        k_cell_table<<<blocks_for(n, 256), 256, 0, s>>>(g.keys[g.sorted_slot], n, zbits, g.cell_beg, g.cell_end);
 }
 
 void launch_permute(const uint32_t* perm, int n, const DevState& src, const DevState& dst, const DevIni* ini_src,

@@ -67,7 +67,7 @@ void launch_hash_table(const RunDesc* d, const int32_t* run_of, const float4* po
 // Stable LSD radix sort of keys[0] with values = iota; result in slot g.sorted_slot.
 void launch_radix_sort(GridBuffers& g, int n, int bits, hipStream_t s);
 // cell_beg / cell_end of the sorted keys (-1 for empty cells).
-void launch_cell_table(GridBuffers& g, int n, int64_t ncell_total, hipStream_t s);
+void launch_cell_table(GridBuffers& g, int n, int64_t ncell_total, int zbits, hipStream_t s);
 // dst[k] = src[perm[k]] for all state arrays (and ini when both set).  When
 // str_dst is non-null the permuted stress goes there instead of dst.str.
 void launch_permute(const uint32_t* perm, int n, const DevState& src, const DevState& dst, const DevIni* ini_src,
@@ -79,9 +79,12 @@ enum Stage { ST_ADAMI = 0, ST_INTERACT = 1, ST_PREDICT = 2, ST_CORRECT = 3, ST_S
 // _one: a single run by value (walls [0, npb), material [npb, np)).
 // _table: all resident runs; walls / mats map a compact thread index to the
 // particle index of the t-th wall / material particle.
-void launch_adami_one(const RunDesc& d, const DevState& st, const int32_t* beg, const int32_t* end, hipStream_t s);
+// variant >= 3 selects the list-based neighbour scan for Adami and Shepard
+// (same sums in the same order as the direct scan).
+void launch_adami_one(const RunDesc& d, const DevState& st, const int32_t* beg, const int32_t* end, int variant,
+                      hipStream_t s);
 void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const int32_t* beg,
-                        const int32_t* end, IndexMap walls, int nwalls, hipStream_t s);
+                        const int32_t* end, IndexMap walls, int nwalls, int variant, hipStream_t s);
 void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates& rt, const int32_t* beg,
                             const int32_t* end, int32_t* nbr, int variant, hipStream_t s);
 void launch_interaction_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt,
@@ -98,9 +101,10 @@ void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run
 // Shepard filter: reads stress from str_src, writes str_dst (walls and runs
 // not due are copied through).
 void launch_shepard_one(const RunDesc& d, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,
-                        const int32_t* beg, const int32_t* end, int n, hipStream_t s);
+                        const int32_t* beg, const int32_t* end, int n, int variant, hipStream_t s);
 void launch_shepard_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const tensor2d* str_src,
-                          tensor2d* str_dst, const int32_t* beg, const int32_t* end, int n, hipStream_t s);
+                          tensor2d* str_dst, const int32_t* beg, const int32_t* end, int n, int variant,
+                          hipStream_t s);
 // Per-run step bookkeeping on the device.
 void launch_step_begin(RunDesc* d, int nruns, hipStream_t s);
 void launch_step_end(RunDesc* d, int nruns, const int32_t* fail, hipStream_t s);

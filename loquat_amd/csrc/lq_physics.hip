@@ -65,11 +65,13 @@ __device__ __forceinline__ tensor2d jaumann(const float4 w, const tensor2d& s) {
 }
 
 // ------------------------------------------------------------ Adami walls
-// n threads over the walls of all runs, walls(t) = particle index.
+// n threads over the walls of all runs, walls(t) = particle index.  Blocks
+// keep the hardware's round-robin XCD order: only walls next to material have
+// work, and contiguous XCD slices would leave most XCDs idle.
 template <class DP>
 __global__ __launch_bounds__(256) void k_adami(DP dp, DevState st, const int32_t* __restrict__ beg,
                                                const int32_t* __restrict__ end, IndexMap walls, int n) {
-    const int t = xcd_thread();
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int k = walls(t);
     const RunDesc& d = dp.get(dp.run(k));
@@ -468,19 +470,28 @@ __device__ __forceinline__ T ldo(const T* __restrict__ base, uint32_t idx) {
     return *(const T*)((const char*)base + (idx * (uint32_t)sizeof(T)));
 }
 
-// Pair body of variant 3: as pair_eval, with 32-bit offset loads and the
-// viscosity term evaluated branch-free.
-__device__ __forceinline__ void pair_eval3(const IPart& pi, uint32_t j, const DevState& st, const IConst& c,
-                                           IAcc& a) {
-    const float dx = (float)(pi.x - ldo(st.xd, j)), dy = (float)(pi.y - ldo(st.yd, j)),
-                dz = (float)(pi.z - ldo(st.zd, j));
+// What a pair reads of particle j.
+struct PairIn {
+    double x, y, z;
+    float4 vr;
+    tensor2d s;
+};
+
+__device__ __forceinline__ PairIn pair_load(const DevState& st, uint32_t j) {
+    return PairIn{ldo(st.xd, j), ldo(st.yd, j), ldo(st.zd, j), ldo(st.velrhop, j), ldo(st.str, j)};
+}
+
+// Pair body of variants 3 and 4: as pair_eval, with 32-bit offset loads and
+// the viscosity term evaluated branch-free.
+__device__ __forceinline__ void pair_compute(const IPart& pi, const PairIn& pj, const IConst& c, IAcc& a) {
+    const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y), dz = (float)(pi.z - pj.z);
     const float r2 = dx * dx + dy * dy + dz * dz;
     const float q = fsqrt(r2) * c.i_h;
     const float t = fmaxf(2.0f - q, 0.0f);
     const float fac = c.cg * t * t * t;
     const float gx = fac * dx, gy = fac * dy, gz = fac * dz;
-    const float4 vr = ldo(st.velrhop, j);
-    const tensor2d sj = ldo(st.str, j);
+    const float4 vr = pj.vr;
+    const tensor2d sj = pj.s;
     const float ir = frcp(vr.w);
     const float dvx = vr.x - pi.vx, dvy = vr.y - pi.vy, dvz = vr.z - pi.vz;
     const float ux = dvx * ir, uy = dvy * ir, uz = dvz * ir;
@@ -509,61 +520,110 @@ __device__ __forceinline__ void pair_eval3(const IPart& pi, uint32_t j, const De
     a.az += sxz * gx + syz * gy + szz * gz;
 }
 
-// Variant 3: variant 2's structure with fewer instructions per candidate
-// and per pair, and XCD-contiguous blocks.  The particle itself stays in its
-// candidate stream (it is always a hit when it lies inside the grid): all its
-// pair terms are exactly zero (x_i - x_i = 0 zeroes the kernel gradient and
-// v_i - v_i the velocity terms), so it changes no sum, and one is taken off
-// the neighbour count.
-template <class DP>
-__global__ __launch_bounds__(256) void k_interaction_v3(DP dp, DevState st, DevRates rt,
-                                                        const int32_t* __restrict__ beg,
-                                                        const int32_t* __restrict__ end, IndexMap mats, int n,
-                                                        int32_t* __restrict__ nbr) {
-    __shared__ int32_t lists[4][IC2_CAP][64];
-    const int t = xcd_thread();
-    const int k = t < n ? mats(t) : -1;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int32_t(*my)[64] = lists[w];
-    bool mat = false;
-    const RunDesc* dptr = nullptr;
-    if (t < n) {
-        const RunDesc& d = dp.get(dp.run(k));
-        dptr = &d;
-        mat = (k - d.p_off) >= d.npb && d.active;
-    }
-    IConst c = {};
-    float r2cut = 0.f;
-    float4 ps = make_float4(0.f, 0.f, 0.f, 0.f);
-    IPart pi = {};
-    int cix = 0, ciy = 0, ciz = 0, nx = 0, ny = 0, nz = 0, nol = 0, cbase = 0;
-    if (mat) {
-        const RunDesc& d = *dptr;
-        c = make_iconst(d);
-        r2cut = d.r2_cut;
-        ps = st.pospres[k];
-        load_part(st, k, pi);
-        cix = cell_coord(ps.x, c.i_h);
-        ciy = cell_coord(ps.y, c.i_h);
-        ciz = cell_coord(ps.z, c.i_h);
-        nx = d.par.cell_num_x;
-        ny = d.par.cell_num_y;
-        nz = d.par.cell_num_z;
-        nol = d.par.ncell_one_layer;
-        cbase = d.cell_base;
-    }
-    const bool self_in = mat && cix >= 0 && cix < nx && ciy >= 0 && ciy < ny && ciz >= 0 && ciz < nz;
-    IAcc a = {};
+__device__ __forceinline__ void pair_eval3(const IPart& pi, uint32_t j, const DevState& st, const IConst& c,
+                                           IAcc& a) {
+    pair_compute(pi, pair_load(st, j), c, a);
+}
+
+// Wave-cooperative neighbour scan shared by the list-based kernels.  For a
+// lane with `active` set it walks the z-merged (x, y) columns of cell layers
+// [layer_lo, layer_hi] around cell (cix, ciy, ciz), tests candidates four at a
+// time with the FP32 predicate (branch-free, four loads in flight), appends
+// hits to the lane's LDS list `my`, and calls eval(j, true) for every listed
+// j, in candidate order, whenever some lane's list is nearly full and at the
+// end.  Lanes with a shorter list meanwhile call eval(dummy, false), so that
+// the accumulators are updated unconditionally (no copies around a masked
+// update); eval must make that call add exactly nothing.  Every lane of the
+// wave must call it (it votes); returns the lane's hits.
+//
+// Pruning (never drops a pair the predicate accepts): a column whose nearest
+// (x, y) distance already exceeds the support is skipped, and when cells are
+// sorted by z-bin (zbits > 0) each column is read only over the z-window
+// |dz| <= sqrt(R^2 - dxy_min^2) widened by one bin: blocks of four lying wholly
+// below it are stepped over with one load, and the scan stops at the first
+// block whose last particle lies above it.  R is the support radius inflated
+// by 1e-3 and dxy_min is shrunk by 1e-3 of a cell, far beyond the FP32
+// rounding of the coordinates, so every skipped candidate has r2 > r2_cut.
+struct ScanGeom {
+    int cix, ciy, ciz, nx, ny, nz, nol, cbase;
+    int zbits;
+    float u, v;     // position inside the cell along x and y, in cell units
+    float cw;       // cell width (2 h)
+    float rw2;      // (1.001 R)^2, R^2 = r2_cut
+    float z, binw;  // particle z and the z-bin width widened by 1 %
+};
+
+__device__ __forceinline__ ScanGeom make_geom(const RunDesc& d, float4 ps) {
+    const float i_h = d.par.i_h;
+    ScanGeom g;
+    g.cix = cell_coord(ps.x, i_h);
+    g.ciy = cell_coord(ps.y, i_h);
+    g.ciz = cell_coord(ps.z, i_h);
+    g.nx = d.par.cell_num_x;
+    g.ny = d.par.cell_num_y;
+    g.nz = d.par.cell_num_z;
+    g.nol = d.par.ncell_one_layer;
+    g.cbase = d.cell_base;
+    g.zbits = d.zbits;
+    const float sx = (0.5f * ps.x) * i_h, sy = (0.5f * ps.y) * i_h;
+    g.u = sx - floorf(sx);
+    g.v = sy - floorf(sy);
+    g.cw = 2.0f / i_h;
+    const float rw = 1.001f * sqrtf(d.r2_cut);
+    g.rw2 = rw * rw;
+    g.z = ps.z;
+    g.binw = 1.01f * g.cw / (float)(1 << d.zbits);
+    return g;
+}
+
+//
+// B = 2 evaluates the list two entries at a time, eval(j0, act0, j1, act1),
+// so that both gathers are in flight before either pair is computed.
+template <int B, bool PADDED, int U = IC2_U, class Eval>
+__device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer_hi, const ScanGeom& g, float4 ps,
+                                            float r2cut, const int32_t* __restrict__ beg,
+                                            const int32_t* __restrict__ end, const float4* __restrict__ pp,
+                                            int32_t (*my)[64], int lane, uint32_t dummy, Eval&& eval) {
+    static_assert(B == 1 || B == 2, "batch of 1 or 2");
     int count = 0, cnt = 0;
-    const int czlo = max(ciz - 1, 0), czhi = min(ciz + 1, nz - 1);
-    const float4* __restrict__ pp = st.pospres;
-    for (int col = 0; col < 18; ++col) {
-        const int layer = col / 9, ox = (col % 9) / 3 - 1, oy = col % 3 - 1;
+    auto flush = [&]() {
+        // bottom-tested loop: the accumulators flow straight from one
+        // iteration to the next (a top-tested loop makes the compiler copy
+        // them every iteration)
+        if (__ballot(cnt > 0)) {
+            int e = 0;
+            do {
+                if constexpr (B == 1) {
+                    const bool act = e < cnt;
+                    const uint32_t jl = (uint32_t)my[e][lane];
+                    eval(act ? jl : dummy, act);
+                } else {
+                    const bool a0 = e < cnt, a1 = e + 1 < cnt;
+                    const uint32_t j0 = (uint32_t)my[e][lane];
+                    const uint32_t j1 = (uint32_t)my[min(e + 1, IC2_CAP - 1)][lane];
+                    eval(a0 ? j0 : dummy, a0, a1 ? j1 : dummy, a1);
+                }
+                e += B;
+            } while (__ballot(e < cnt));
+        }
+        count += cnt;
+        cnt = 0;
+    };
+    const float* __restrict__ pz = reinterpret_cast<const float*>(pp) + 2;  // pz[4 j] = z of particle j
+    const int czlo = max(g.ciz - 1, 0), czhi = min(g.ciz + 1, g.nz - 1);
+    const int ncol = 9 * (layer_hi - layer_lo + 1);
+    const float margin = 1e-3f * g.cw;
+    for (int col = 0; col < ncol; ++col) {
+        const int layer = layer_lo + col / 9, ox = (col % 9) / 3 - 1, oy = col % 3 - 1;
         int j = 0, je = 0;
-        if (mat && czlo <= czhi) {
-            const int cx = cix + ox, cy = ciy + oy;
-            if (cx >= 0 && cx < nx && cy >= 0 && cy < ny) {
-                const int c0 = cbase + layer * nol + (cx * ny + cy) * nz;
+        float zhi = INFINITY;
+        if (active && czlo <= czhi) {
+            const int cx = g.cix + ox, cy = g.ciy + oy;
+            const float ex = ox == 0 ? 0.0f : fmaxf((ox < 0 ? g.u : 1.0f - g.u) * g.cw - margin, 0.0f);
+            const float ey = oy == 0 ? 0.0f : fmaxf((oy < 0 ? g.v : 1.0f - g.v) * g.cw - margin, 0.0f);
+            const float hz2 = g.rw2 - (ex * ex + ey * ey);
+            if (hz2 > 0.0f && cx >= 0 && cx < g.nx && cy >= 0 && cy < g.ny) {
+                const int c0 = g.cbase + layer * g.nol + (cx * g.ny + cy) * g.nz;
                 int b0 = -1, e0 = -1;
                 for (int cz = czlo; cz <= czhi; ++cz) {
                     const int b = beg[c0 + cz];
@@ -575,52 +635,198 @@ __global__ __launch_bounds__(256) void k_interaction_v3(DP dp, DevState st, DevR
                 if (b0 >= 0) {
                     j = b0;
                     je = e0;
+                    if (g.zbits) {
+                        const float hz = sqrtf(hz2);
+                        const float zlo = g.z - hz - g.binw;
+                        zhi = g.z + hz + g.binw;
+                        while (j + U <= je && ldo(pz, 4u * (uint32_t)(j + U - 1)) < zlo) j += U;
+                    }
                 }
             }
         }
         const uint32_t jlast = (uint32_t)max(je - 1, 0);
         while (true) {
-            while (__ballot(j < je) && !__ballot(cnt > IC2_CAP - IC2_U)) {
-                float4 q[IC2_U];
+            while (__ballot(j < je) && !__ballot(cnt > IC2_CAP - U)) {
+                float4 q[U];
+                if (PADDED) {
+                    // one offset for the four loads; a finished lane re-reads
+                    // [je, je + 3], inside the padded arrays
+                    const uint32_t jj = (uint32_t)min(j, je);
 #pragma unroll
-                for (int u = 0; u < IC2_U; ++u) q[u] = ldo(pp, min((uint32_t)(j + u), jlast));
-                int hit[IC2_U];
+                    for (int u = 0; u < U; ++u) q[u] = ldo(pp + u, jj);
+                } else {
 #pragma unroll
-                for (int u = 0; u < IC2_U; ++u) {
+                    for (int u = 0; u < U; ++u) q[u] = ldo(pp, min((uint32_t)(j + u), jlast));
+                }
+                int hit[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
                     // both tests evaluated, combined without a branch
                     const int close = r2_fp32(ps.x - q[u].x, ps.y - q[u].y, ps.z - q[u].z) < r2cut ? 1 : 0;
                     const int inside = (j + u < je) ? 1 : 0;
                     hit[u] = close & inside;
                 }
 #pragma unroll
-                for (int u = 0; u < IC2_U; ++u) {
+                for (int u = 0; u < U; ++u) {
                     my[cnt][lane] = j + u;
                     cnt += hit[u];
                 }
-                j += IC2_U;
+                j += U;
+                // every later particle of the column lies above the window
+                if (q[U - 1].z > zhi) je = min(je, j);
             }
             if (!__ballot(j < je)) break;
-            for (int e = 0; __ballot(e < cnt); ++e) {
-                if (e < cnt) pair_eval3(pi, (uint32_t)my[e][lane], st, c, a);
-            }
-            count += cnt;
-            cnt = 0;
+            flush();
         }
-        if (__ballot(cnt > IC2_CAP - 3 * IC2_U)) {
-            for (int e = 0; __ballot(e < cnt); ++e) {
-                if (e < cnt) pair_eval3(pi, (uint32_t)my[e][lane], st, c, a);
-            }
-            count += cnt;
-            cnt = 0;
+        // flush between columns when another column might not fit
+        if (__ballot(cnt > IC2_CAP - (U == 4 ? 3 * U : 2 * U))) flush();
+    }
+    flush();
+    return count;
+}
+
+// Variant 3 (B = 1): variant 2's structure with fewer instructions per
+// candidate and per pair, pruned scans, and XCD-contiguous blocks.  The
+// particle itself stays in its candidate stream (it is always a hit when it
+// lies inside the grid): all its pair terms are exactly zero (x_i - x_i = 0
+// zeroes the kernel gradient and v_i - v_i the velocity terms), so it changes
+// no sum, and one is taken off the neighbour count.  Variant 4 (B = 2)
+// evaluates the lists two pairs at a time; same pairs, same order.
+template <class DP, int B, bool MASKED = false, int MINW = 1, int U = IC2_U>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(MINW))) void k_interaction_v3(
+    DP dp, DevState st, DevRates rt,
+                                                        const int32_t* __restrict__ beg,
+                                                        const int32_t* __restrict__ end, IndexMap mats, int n,
+                                                        int32_t* __restrict__ nbr) {
+    __shared__ int32_t lists[4][IC2_CAP][64];
+    const int t = xcd_thread();
+    const int k = t < n ? mats(t) : -1;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    bool mat = false;
+    const RunDesc* dptr = nullptr;
+    if (t < n) {
+        const RunDesc& d = dp.get(dp.run(k));
+        dptr = &d;
+        mat = (k - d.p_off) >= d.npb && d.active;
+    }
+    IConst c = {};
+    float r2cut = 0.f;
+    float4 ps = make_float4(0.f, 0.f, 0.f, 0.f);
+    IPart pi = {};
+    ScanGeom g = {};
+    if (mat) {
+        const RunDesc& d = *dptr;
+        c = make_iconst(d);
+        r2cut = d.r2_cut;
+        ps = st.pospres[k];
+        load_part(st, k, pi);
+        g = make_geom(d, ps);
+    }
+    const bool self_in =
+        mat && g.cix >= 0 && g.cix < g.nx && g.ciy >= 0 && g.ciy < g.ny && g.ciz >= 0 && g.ciz < g.nz;
+    IAcc a = {};
+    // dummy = the particle itself: its pair terms are exactly zero
+    int count;
+    // MASKED: a lane past the end of its list loads nothing and computes the
+    // self pair from registers (positions and velocity of i, zero stress),
+    // instead of gathering particle i's data again
+    const PairIn self = PairIn{pi.x, pi.y, pi.z, make_float4(pi.vx, pi.vy, pi.vz, pi.rho), tensor2d{}};
+    auto load = [&](uint32_t j, bool act) {
+        if constexpr (MASKED) {
+            PairIn q = self;
+            if (act) q = pair_load(st, j);
+            return q;
+        } else {
+            return pair_load(st, j);
         }
+    };
+    if constexpr (B == 1) {
+        count = scan_compact<1, DP::padded, U>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+                                            mat ? (uint32_t)k : 0u,
+                                            [&](uint32_t j, bool act) { pair_compute(pi, load(j, act), c, a); });
+    } else {
+        count = scan_compact<2, DP::padded, U>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+                                            mat ? (uint32_t)k : 0u,
+                                            [&](uint32_t j0, bool a0, uint32_t j1, bool a1) {
+                                                const PairIn q0 = load(j0, a0);
+                                                const PairIn q1 = load(j1, a1);
+                                                pair_compute(pi, q0, c, a);
+                                                pair_compute(pi, q1, c, a);
+                                            });
     }
-    for (int e = 0; __ballot(e < cnt); ++e) {
-        if (e < cnt) pair_eval3(pi, (uint32_t)my[e][lane], st, c, a);
-    }
-    count += cnt;
     if (mat) {
         store_rates(*dptr, rt, k, a);
         if (nbr) nbr[k] = count - (self_in ? 1 : 0);
+    }
+}
+
+// Adami wall stress with the list-based scan (material layer only; a wall is
+// never in its own candidate stream).  Same sums as k_adami, same order.
+template <class DP>
+__global__ __launch_bounds__(256) void k_adami_lds(DP dp, DevState st, const int32_t* __restrict__ beg,
+                                                   const int32_t* __restrict__ end, IndexMap walls, int n) {
+    __shared__ int32_t lists[4][IC2_CAP][64];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;  // round-robin XCDs, as k_adami
+    const int k = t < n ? walls(t) : -1;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    bool on = false;
+    const RunDesc* dptr = nullptr;
+    if (t < n) {
+        const RunDesc& d = dp.get(dp.run(k));
+        dptr = &d;
+        on = (k - d.p_off) < d.npb && d.active;
+    }
+    float i_h = 0.f, cw = 0.f, m = 0.f, r2cut = 0.f;
+    float4 ps = make_float4(0.f, 0.f, 0.f, 0.f);
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    ScanGeom g = {};
+    if (on) {
+        const RunDesc& d = *dptr;
+        i_h = d.par.i_h;
+        cw = d.par.kernel_normalization_par;
+        m = d.par.m;
+        r2cut = d.r2_cut;
+        ps = st.pospres[k];
+        xi = st.xd[k];
+        yi = st.yd[k];
+        zi = st.zd[k];
+        g = make_geom(d, ps);
+    }
+    float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
+    float px = 0.f, py = 0.f, pz = 0.f, wsum = 0.f;
+    scan_compact<1, DP::padded>(on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane, on ? (uint32_t)k : 0u,
+                             [&](uint32_t j, bool act) {
+        const float dx = (float)(xi - ldo(st.xd, j)), dy = (float)(yi - ldo(st.yd, j)),
+                    dz = (float)(zi - ldo(st.zd, j));
+        const float q = fsqrt(dx * dx + dy * dy + dz * dz) * i_h;
+        const float tt = fmaxf(2.0f - q, 0.0f);
+        const float t2 = tt * tt;
+        const float kv0 = cw * t2 * t2 * (q + 0.5f) * m * frcp(ldo(st.velrhop, j).w);
+        const float kv = act ? kv0 : 0.0f;
+        const tensor2d s = ldo(st.str, j);
+        sxx += s.xx * kv;
+        syy += s.yy * kv;
+        szz += s.zz * kv;
+        sxy += s.xy * kv;
+        sxz += s.xz * kv;
+        syz += s.yz * kv;
+        px += dx * kv;
+        py += dy * kv;
+        pz += dz * kv;
+        wsum += kv;
+    });
+    if (on && wsum > 0.0f) {
+        const RunDesc& d = *dptr;
+        const float iw = 1.0f / wsum;
+        const float rho0 = d.par.rho0;
+        tensor2d o;
+        o.xx = (sxx - rho0 * d.par.acc_x * px) * iw;
+        o.yy = (syy - rho0 * d.par.acc_y * py) * iw;
+        o.zz = (szz - rho0 * d.par.acc_z * pz) * iw;
+        o.xy = sxy * iw;
+        o.xz = sxz * iw;
+        o.yz = syz * iw;
+        st.str[k] = o;
     }
 }
 
@@ -936,6 +1142,62 @@ __global__ __launch_bounds__(256) void k_shepard(DP dp, DevState st, const tenso
     }
 }
 
+// Shepard filter with the list-based scan; same sums as k_shepard, same order.
+template <class DP>
+__global__ __launch_bounds__(256) void k_shepard_lds(DP dp, DevState st, const tensor2d* __restrict__ src,
+                                                     tensor2d* __restrict__ dst, const int32_t* __restrict__ beg,
+                                                     const int32_t* __restrict__ end, int n) {
+    __shared__ int32_t lists[4][IC2_CAP][64];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;  // walls copy only: round-robin XCDs balance the work
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    bool on = false;
+    float i_h = 0.f, cw = 0.f, m = 0.f, r2cut = 0.f;
+    float4 ps = make_float4(0.f, 0.f, 0.f, 0.f);
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    ScanGeom g = {};
+    if (k < n) {
+        const RunDesc& d = dp.get(dp.run(k));
+        on = (k - d.p_off) >= d.npb && d.active && d.shepard_due;
+        if (on) {
+            i_h = d.par.i_h;
+            cw = d.par.kernel_normalization_par;
+            m = d.par.m;
+            r2cut = d.r2_cut;
+            ps = st.pospres[k];
+            xi = st.xd[k];
+            yi = st.yd[k];
+            zi = st.zd[k];
+            g = make_geom(d, ps);
+        }
+    }
+    float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f, wsum = 0.f;
+    scan_compact<1, DP::padded>(on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane, on ? (uint32_t)k : 0u,
+                             [&](uint32_t j, bool act) {
+        const float dx = (float)(xi - ldo(st.xd, j)), dy = (float)(yi - ldo(st.yd, j)),
+                    dz = (float)(zi - ldo(st.zd, j));
+        const float q = fsqrt(dx * dx + dy * dy + dz * dz) * i_h;
+        const float tt = fmaxf(2.0f - q, 0.0f);
+        const float t2 = tt * tt;
+        const float kv0 = cw * t2 * t2 * (q + 0.5f) * m * frcp(ldo(st.velrhop, j).w);
+        const float kv = act ? kv0 : 0.0f;
+        const tensor2d s = ldo(src, j);
+        sxx += s.xx * kv;
+        syy += s.yy * kv;
+        szz += s.zz * kv;
+        sxy += s.xy * kv;
+        sxz += s.xz * kv;
+        syz += s.yz * kv;
+        wsum += kv;
+    });
+    if (k >= n) return;
+    if (on && wsum > 0.0f) {
+        const float iw = 1.0f / wsum;
+        dst[k] = tensor2d{sxx * iw, syy * iw, szz * iw, sxy * iw, sxz * iw, syz * iw};
+    } else {
+        dst[k] = src[k];
+    }
+}
+
 __global__ void k_step_begin(RunDesc* d, int nruns) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nruns) return;
@@ -965,14 +1227,23 @@ __global__ void k_copy_tensor(const tensor2d* __restrict__ src, tensor2d* __rest
 }  // namespace
 
 // One run: walls are [0, npb), material particles [npb, np).
-void launch_adami_one(const RunDesc& d, const DevState& st, const int32_t* beg, const int32_t* end, hipStream_t s) {
-    const int n = d.npb;
-    if (n > 0) k_adami<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, beg, end, IndexMap{nullptr, 0}, n);
+// Variants 3 and up use the list-based neighbour scan, the others the direct one.
+template <class DP>
+static void launch_adami(const DP& dp, const DevState& st, const int32_t* beg, const int32_t* end, IndexMap walls,
+                         int n, int variant, hipStream_t s) {
+    if (n <= 0) return;
+    if (variant >= 3)
+        k_adami_lds<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, beg, end, walls, n);
+    else
+        k_adami<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, beg, end, walls, n);
+}
+void launch_adami_one(const RunDesc& d, const DevState& st, const int32_t* beg, const int32_t* end, int variant,
+                      hipStream_t s) {
+    launch_adami(DescOne{d}, st, beg, end, IndexMap{nullptr, 0}, d.npb, variant, s);
 }
 void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const int32_t* beg,
-                        const int32_t* end, IndexMap walls, int nwalls, hipStream_t s) {
-    if (nwalls > 0)
-        k_adami<DescTable><<<blocks_for(nwalls, 256), 256, 0, s>>>(DescTable{d, run_of}, st, beg, end, walls, nwalls);
+                        const int32_t* end, IndexMap walls, int nwalls, int variant, hipStream_t s) {
+    launch_adami(DescTable{d, run_of}, st, beg, end, walls, nwalls, variant, s);
 }
 
 template <class DP>
@@ -986,8 +1257,14 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
         k_interaction_lds<DP><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else if (variant == 2)
         k_interaction_unrolled<DP><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+    else if (variant == 3)
+        k_interaction_v3<DP, 1><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+    else if (variant == 4)
+        k_interaction_v3<DP, 2><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+    else if (variant == 5)
+        k_interaction_v3<DP, 1, true><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else
-        k_interaction_v3<DP><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+        k_interaction_v3<DP, 2, true><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
 }
 
 void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates& rt, const int32_t* beg,
@@ -1021,15 +1298,23 @@ void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run
         k_predictor<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat);
 }
 
+template <class DP>
+static void launch_shepard(const DP& dp, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,
+                           const int32_t* beg, const int32_t* end, int n, int variant, hipStream_t s) {
+    if (n <= 0) return;
+    if (variant >= 3)
+        k_shepard_lds<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, str_src, str_dst, beg, end, n);
+    else
+        k_shepard<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, str_src, str_dst, beg, end, n);
+}
 void launch_shepard_one(const RunDesc& d, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,
-                        const int32_t* beg, const int32_t* end, int n, hipStream_t s) {
-    if (n > 0) k_shepard<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, str_src, str_dst, beg, end, n);
+                        const int32_t* beg, const int32_t* end, int n, int variant, hipStream_t s) {
+    launch_shepard(DescOne{d}, st, str_src, str_dst, beg, end, n, variant, s);
 }
 void launch_shepard_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const tensor2d* str_src,
-                          tensor2d* str_dst, const int32_t* beg, const int32_t* end, int n, hipStream_t s) {
-    if (n > 0)
-        k_shepard<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, st, str_src, str_dst, beg, end,
-                                                                 n);
+                          tensor2d* str_dst, const int32_t* beg, const int32_t* end, int n, int variant,
+                          hipStream_t s) {
+    launch_shepard(DescTable{d, run_of}, st, str_src, str_dst, beg, end, n, variant, s);
 }
 
 void launch_step_begin(RunDesc* d, int nruns, hipStream_t s) {

@@ -219,6 +219,16 @@ class Engine:
     def set_graphs(self, on: bool) -> None:
         abi.check(abi.lib().lq_engine_set_graphs(self.h, 1 if on else 0))
 
+    def set_cell_order(self, zbin_bits: int) -> None:
+        """0: the reference's in-cell order; b > 0: cells sorted by 2^b z-bins."""
+        abi.check(abi.lib().lq_engine_set_cell_order(self.h, int(zbin_bits)))
+
+    def neighbour_counts(self, rid: int) -> np.ndarray:
+        """Interaction neighbour count per particle of run rid (input order)."""
+        out = np.zeros(self.info(rid)["np"], np.int32)
+        abi.check(abi.lib().lq_engine_neighbour_counts(self.h, rid, out.ctypes.data))
+        return out
+
     def info(self, rid: int) -> dict:
         s, t, f, n = C.c_int64(), C.c_double(), C.c_int32(), C.c_int32()
         abi.check(abi.lib().lq_engine_run_info(self.h, rid, C.byref(s), C.byref(t), C.byref(f), C.byref(n)))
@@ -242,7 +252,7 @@ class Engine:
     def autotune(self, nrep: int = 3) -> tuple[int, list[float]]:
         """Pins the fastest interaction variant for this engine."""
         best = C.c_int32()
-        ms = (C.c_double * 4)()
+        ms = (C.c_double * abi.INTERACTION_VARIANTS)()
         abi.check(abi.lib().lq_engine_autotune(self.h, int(nrep), C.byref(best), ms))
         return best.value, list(ms)
 

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from helpers import rel_err
-from loquat_amd import cases
+from loquat_amd import abi, cases
 from oracle import lq_oracle as O
 
 
@@ -115,15 +115,19 @@ def test_interaction_variants_agree_in_engine(gpu):
     from loquat_amd import abi
     c = cases.perturb(cases.small_column_3d(seed=41), seed=42)
     res = []
-    for v in (0, 3):
+    for v in (0, 3, 4, 5, 6):
         abi.check(abi.lib().lq_set_interaction_variant(v))
         e, (rid,) = _engine([c], 5)
         e.step(5)
         res.append(e.download(rid))
         e.close()
     abi.check(abi.lib().lq_set_interaction_variant(3))
-    for k in ("posd", "velrhop", "str"):
-        assert rel_err(res[0][k], res[1][k]) < 1e-5
+    for r in res[1:]:
+        for k in ("posd", "velrhop", "str"):
+            assert rel_err(res[0][k], r[k]) < 1e-5
+    # variants 3 to 6 evaluate the same pairs in the same order
+    for r in res[2:]:
+        assert _same(res[1], r)
 
 
 @pytest.mark.gpu
@@ -131,7 +135,7 @@ def test_autotune_picks_a_variant_and_keeps_results(gpu):
     c = cases.perturb(cases.small_column_3d(seed=51), seed=52)
     e, (rid,) = _engine([c], 6)
     best, ms = e.autotune(2)
-    assert best in (0, 1, 2, 3) and all(m > 0 for m in ms) and ms[best] == min(ms)
+    assert best in range(abi.INTERACTION_VARIANTS) and all(m > 0 for m in ms) and ms[best] == min(ms)
     e.step(6)
     tuned = e.download(rid)
     e.close()
@@ -173,3 +177,51 @@ def test_c1_slab_full_run_matches_oracle(gpu):
     diffs = [abs(a - b) for a, b in zip(got["profile"][:n], gold["profile"][:n]) if a is not None and b is not None]
     assert len(diffs) >= 0.8 * n
     assert max(diffs) <= 0.01 * h0, max(diffs)
+
+
+def _engine_ordered(case, zbits, steps):
+    from loquat_amd.device import Engine
+    e = Engine(case.np, case.derived().ncell, max_runs=1)
+    e.set_cell_order(zbits)
+    return e, e.add(case, steps)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("zbits", [0, 4])
+def test_engine_neighbour_counts_exact(gpu, zbits):
+    """Pruned neighbour scans (column skip, and the z-window over z-binned
+    cells when zbits > 0) find exactly the oracle's neighbours, particle by
+    particle, on the engine's own moved state."""
+    case = cases.perturb(cases.small_column_3d(seed=61), seed=62)
+    e, rid = _engine_ordered(case, zbits, 4)
+    e.step(3)
+    got = e.neighbour_counts(rid)
+    h = e.download(rid)
+    e.close()
+    p = case.derived()
+    st = O.OState.from_arrays(h["posd"], h["pospres"], h["velrhop"], h["idep"], h["str"], h["stn"])
+    O.initial_sort(p, st)
+    g = O.grid_build(p, st)
+    _, nbr = O.interaction(p, st, g["cell_beg"], g["cell_end"])
+    ref = np.zeros_like(nbr)
+    ref[st.orig] = nbr
+    npb = p.npb
+    assert (got[:npb] == 0).all() and got[npb:].min() > 0
+    assert np.array_equal(got[npb:], ref[npb:])
+
+
+@pytest.mark.gpu
+def test_cell_order_changes_only_sum_order(gpu):
+    """z-binned and reference in-cell order: same neighbour sets, so the
+    trajectories differ only by FP32 summation order."""
+    case = cases.perturb(cases.small_column_3d(seed=71, shepard=2), seed=72)
+    res = []
+    for zb in (0, 4):
+        e, rid = _engine_ordered(case, zb, 5)
+        e.step(5)
+        res.append(e.download(rid))
+        e.close()
+    x0 = case.posd
+    assert rel_err(res[0]["posd"] - x0, res[1]["posd"] - x0) < 1e-5
+    for k in ("velrhop", "str"):
+        assert rel_err(res[0][k], res[1][k]) < 1e-5, k

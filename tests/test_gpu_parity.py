@@ -147,7 +147,7 @@ def test_adami_parity(gpu):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("which", ["small3d", "c1"])
 def test_interaction_parity(gpu, variant, which):
     dev = _dev()
