@@ -273,7 +273,7 @@ def main() -> None:
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variant", type=int, default=3, help="interaction kernel variant (without --autotune)")
+    ap.add_argument("--variant", type=int, default=None, help="interaction kernel variant (without --autotune)")
     ap.add_argument("--autotune", type=int, default=1, help="1: time every interaction variant on the "
                     "workload's state and run the fastest")
     ap.add_argument("--ensemble-runs", type=int, default=256)
@@ -299,6 +299,8 @@ def main() -> None:
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if a.variant is None:
+        a.variant = abi.DEFAULT_INTERACTION_VARIANT
     abi.check(abi.lib().lq_set_interaction_variant(a.variant))
 
     case = cases.column_3d(nsteps=a.warmup + a.steps + a.profile_steps + 1)

@@ -150,6 +150,29 @@ lq_status lq_load_parameters(const char* prefix, lq_params* par, lq_vw_params* v
  * columns from <prefix>.dat into caller-allocated host arrays. */
 lq_status lq_load_particles(const char* prefix, const lq_params* par, lq_host_particles* out);
 
+/* LoquatIo::SaveXml + SaveParticles (src/loquat_io.cu:240-549): writes
+ * <dir>/output_<N>.xml and <dir>/<N>.dat (N = par->output_number; the
+ * reference's 22-column format), a binary <dir>/<N>.rst with what the .dat
+ * cannot hold (FP64 positions -- posd, or the FP32 ones when posd is NULL --
+ * and the model-3 void ratio idep.z), and with output_format 2 one
+ * <dir>/<id>_<N>.vtu per object id.  h holds the particles in input order.
+ * Every output is a restart (lq_load_parameters("<dir>/output_<N>"),
+ * lq_load_particles("<dir>/<N>"), lq_load_restart("<dir>/<N>")), which the
+ * reference's is not; the differences:
+ *   - the DP shear modulus is written as elastic-G, the key the reader
+ *     expects (the reference writes poisson-G, src/loquat_io.cu:272 vs :153);
+ *   - the .dat name has no trailing newline (src/loquat_io.cu:354);
+ *   - the .dat is written for output_format 2 as well;
+ *   - numbers carry 9 significant digits, so they read back as the same
+ *     floats (the reference prints 6);
+ *   - model 3 writes its vw-* constants (vw may be NULL otherwise). */
+lq_status lq_save_output(const char* dir, const lq_params* par, const lq_vw_params* vw, const lq_host_particles* h,
+                         const double* posd);
+
+/* Reads <prefix>.rst written by lq_save_output: posd (3*np doubles) and, when
+ * idep is not NULL, idep[4*i+2] (void ratio) of each particle. */
+lq_status lq_load_restart(const char* prefix, const lq_params* par, double* posd, float* idep);
+
 /* Last error message of the calling thread (empty if none). */
 const char* lq_last_error(void);
 
@@ -298,16 +321,18 @@ lq_status lq_engine_profile(lq_engine* eng, int32_t nsteps, double* ms_stage5);
  * (nullable, LQ_INTERACTION_VARIANTS entries) receives the mean launch time
  * of each variant. */
 #define LQ_INTERACTION_VARIANTS 7
+#define LQ_DEFAULT_INTERACTION_VARIANT 6
 lq_status lq_engine_autotune(lq_engine* eng, int32_t nrep, int32_t* best, double* ms_per_variant);
 
 /* Interaction kernel variant for the engine and lq_particle_interaction:
  * 0 = per-thread direct loop, 1 = LDS-compacted neighbour lists,
  * 2 = LDS lists with a 4-wide unrolled candidate scan,
  * 3 = variant 2 with offset loads, pruned column scans, XCD-contiguous
- *     blocks (default),
+ *     blocks,
  * 4 = variant 3 evaluating the lists two pairs at a time,
  * 5, 6 = variants 3, 4 where a lane past the end of its list loads nothing
- *        (exec-masked gathers) instead of re-reading its own particle.
+ *        (exec-masked gathers) instead of re-reading its own particle
+ *        (6 is the default).
  * Variants 3 and up also select the list-based Adami and Shepard scans. */
 lq_status lq_set_interaction_variant(int32_t variant);
 int32_t lq_get_interaction_variant(void);

@@ -87,6 +87,7 @@ class LqServerConfig(C.Structure):
 
 REQ_STATE = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "rejected"}
 INTERACTION_VARIANTS = 7  # LQ_INTERACTION_VARIANTS
+DEFAULT_INTERACTION_VARIANT = 6  # LQ_DEFAULT_INTERACTION_VARIANT
 
 
 class LqError(RuntimeError):
@@ -107,6 +108,8 @@ _PROTOS = {
     "lq_load_parameters": (i32, [C.c_char_p, C.POINTER(LqParams), C.POINTER(LqVw)]),
     "lq_load_particles": (i32, [C.c_char_p, C.POINTER(LqParams), C.POINTER(LqHostParticles)]),
     "lq_last_error": (C.c_char_p, []),
+    "lq_save_output": (i32, [C.c_char_p, C.POINTER(LqParams), C.POINTER(LqVw), C.POINTER(LqHostParticles), vp]),
+    "lq_load_restart": (i32, [C.c_char_p, C.POINTER(LqParams), vp, vp]),
     "lq_grid_create": (i32, [C.c_int64, C.c_int64, C.POINTER(vp)]),
     "lq_grid_destroy": (i32, [vp]),
     "lq_grid_build": (i32, [vp, C.POINTER(LqParams), vp, vp]),
@@ -208,6 +211,22 @@ def load_parameters(prefix: str) -> tuple[LqParams, LqVw]:
     p, v = LqParams(), LqVw()
     check(lib().lq_load_parameters(prefix.encode(), C.byref(p), C.byref(v)))
     return p, v
+
+
+def save_output(directory: str, par: LqParams, vw: LqVw | None, arrs: dict, posd=None) -> None:
+    """lq_save_output: arrs holds pospres/velrhop/idep/str/stn in input order."""
+    keep = [np.ascontiguousarray(arrs[k], np.float32) for k in ("pospres", "velrhop", "idep", "str", "stn")]
+    hp = host_particles(*keep)  # keep holds the arrays alive across the call
+    pd = None if posd is None else np.ascontiguousarray(posd, np.float64)
+    check(lib().lq_save_output(directory.encode(), C.byref(par), C.byref(vw) if vw is not None else None,
+                               C.byref(hp), ptr(pd)))
+
+
+def load_restart(prefix: str, par: LqParams, idep=None) -> np.ndarray:
+    """lq_load_restart: FP64 positions (np, 3); writes the void ratio into idep[:, 2] when given."""
+    posd = np.zeros((par.np, 3), np.float64)
+    check(lib().lq_load_restart(prefix.encode(), C.byref(par), ptr(posd), ptr(idep)))
+    return posd
 
 
 def load_particles(prefix: str, par: LqParams) -> dict:

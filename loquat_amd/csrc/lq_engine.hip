@@ -25,7 +25,7 @@ namespace lq {
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 
-static int g_variant = 3;
+static int g_variant = LQ_DEFAULT_INTERACTION_VARIANT;
 int interaction_variant() { return g_variant; }
 
 // Smallest FP32 value T with sqrtf(T) * i_h >= 2 (both correctly rounded):
@@ -194,9 +194,7 @@ extern "C" lq_status lq_grid_build(lq_grid* grid, const lq_params* par, const fl
     }
     hipStream_t s = (hipStream_t)stream;
     const RunDesc d = one_desc(par, nullptr);
-    launch_hash_one(d, (const float4*)pospres, par->np, grid->g.keys[0], s);
-    launch_radix_sort(grid->g, par->np, rs_bits(par->ncell), s);
-    launch_cell_table(grid->g, par->np, par->ncell, 0, s);
+    launch_grid_build_one(d, (const float4*)pospres, grid->g, s);
     grid->n = par->np;
     return last_launch("lq_grid_build");
 }
@@ -399,10 +397,7 @@ enum { PG_GRID = 0, PG_ADAMI = 1, PG_INTER = 2, PG_INTEG = 3, PG_SHEP = 4 };
 void enqueue_build(lq_engine* e, const DevState& st, hipStream_t s) {
     Buffers& b = e->b;
     const int n = (int)e->np_total;
-    launch_hash_table(b.desc, b.run_of, st.pospres, n, b.grid.keys[0], s);
-    const int zb = zbits_eff(e);
-    launch_radix_sort(b.grid, n, rs_bits(e->ncell_total) + zb, s);
-    launch_cell_table(b.grid, n, e->ncell_total, zb, s);
+    launch_grid_build_table(b.desc, b.run_of, st.pospres, n, e->ncell_total, zbits_eff(e), b.grid, s);
 }
 
 // One full step for every resident run, starting from buffer e->cur (the

@@ -8,8 +8,10 @@
 // stable order, so it matches any stable sort of the same keys bit for bit.
 //
 // Radix sort, per 8-bit digit pass (ceil(bits/8) passes, bits = ceil(log2
-// ncell)):
-//   count   : one workgroup per 4096-key tile; each wave ranks 64 keys per
+// ncell) + the z-bin bits of the engine's key):
+//   hash    : the first pass's count is fused with the key computation and
+//             with clearing the cell table (k_hash_count).
+//   count   : one workgroup per 2048-key tile; each wave ranks 64 keys per
 //             round with 8 ballots (wave-wide match of the digit), the lowest
 //             lane of each peer group bumps the wave's LDS counter -- no LDS
 //             atomics, so runs of equal digits (sorted particles share cells)
@@ -169,27 +171,61 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* __
     }
 }
 
-// Cell id of every particle: layer 0 for walls, 1 for material (index rule of
-// src/adami_boundary.cu:73 and src/loquat.cu:119-124); any axis outside the
-// grid sends the particle to ncell_delete = ncell - 1.
+// Sort key of particle k: its cell id (layer 0 for walls, 1 for material,
+// index rule of src/adami_boundary.cu:73 and src/loquat.cu:119-124; any axis
+// outside the grid sends the particle to ncell_delete = ncell - 1), shifted
+// left by zbits with the particle's z-bin inside the cell below it.
 template <class DP>
-__global__ __launch_bounds__(256) void k_hash(DP dp, const float4* __restrict__ pospres, int n,
-                                              uint32_t* __restrict__ keys) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+__device__ __forceinline__ uint32_t cell_key(const DP& dp, int k, float4 p) {
     const RunDesc& d = dp.get(dp.run(k));
     const int layer = (k - d.p_off) < d.npb ? 0 : 1;
-    const float4 p = pospres[k];
     const float i_h = d.par.i_h;
     const int cx = cell_coord(p.x, i_h), cy = cell_coord(p.y, i_h), cz = cell_coord(p.z, i_h);
     const int nx = d.par.cell_num_x, ny = d.par.cell_num_y, nz = d.par.cell_num_z;
-    uint32_t key;
     if (cx < 0 || cx >= nx || cy < 0 || cy >= ny || cz < 0 || cz >= nz)
-        key = (uint32_t)(d.cell_base + d.par.ncell - 1) << d.zbits;
-    else
-        key = ((uint32_t)(d.cell_base + layer * d.par.ncell_one_layer + (cx * ny + cy) * nz + cz) << d.zbits) |
-              (d.zbits ? zbin_in_cell(p.z, i_h, d.zbits, cz) : 0u);
-    keys[k] = key;
+        return (uint32_t)(d.cell_base + d.par.ncell - 1) << d.zbits;
+    return ((uint32_t)(d.cell_base + layer * d.par.ncell_one_layer + (cx * ny + cy) * nz + cz) << d.zbits) |
+           (d.zbits ? zbin_in_cell(p.z, i_h, d.zbits, cz) : 0u);
+}
+
+// Hash fused with the first radix pass's count: one workgroup per tile
+// computes the keys of its tile (stored for the scatter), counts their low
+// digit as k_radix_count does, and clears its share of the cell table
+// (cell_beg = cell_end = -1; the occupied cells are written after the sort).
+template <class DP>
+__global__ __launch_bounds__(RS_THREADS) void k_hash_count(DP dp, const float4* __restrict__ pospres, int n,
+                                                            uint32_t* __restrict__ keys, uint32_t* __restrict__ hist,
+                                                            int ntiles, int32_t* __restrict__ cbeg,
+                                                            int32_t* __restrict__ cend, int64_t ncell) {
+    __shared__ uint32_t cnt[4][RS_BINS];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    for (int i = t; i < 4 * RS_BINS; i += RS_THREADS) (&cnt[0][0])[i] = 0;
+    {
+        const int64_t per = (ncell + ntiles - 1) / ntiles;
+        const int64_t c0 = (int64_t)blockIdx.x * per, c1 = min(c0 + per, ncell);
+        for (int64_t c = c0 + t; c < c1; c += RS_THREADS) {
+            cbeg[c] = -1;
+            cend[c] = -1;
+        }
+    }
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE + w * WAVE_CHUNK;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+#pragma unroll 4
+    for (int r = 0; r < RS_IPT; ++r) {
+        const int idx = base + r * 64 + lane;
+        const bool valid = idx < n;
+        uint32_t key = 0u;
+        if (valid) {
+            key = cell_key(dp, idx, pospres[idx]);
+            keys[idx] = key;
+        }
+        const uint32_t d = key & 255u;
+        const unsigned long long peers = match8(d, valid);
+        if (valid && (peers & lt) == 0ull) cnt[w][d] += (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    hist[(size_t)t * ntiles + blockIdx.x] = cnt[0][t] + cnt[1][t] + cnt[2][t] + cnt[3][t];
 }
 
 __global__ void k_fill2(int32_t* __restrict__ a, int32_t* __restrict__ b, int64_t n, int32_t v) {
@@ -283,28 +319,15 @@ void free_grid(GridBuffers& g) {
     g.cell_beg = g.cell_end = nullptr;
 }
 
-void launch_hash_one(const RunDesc& d, const float4* pospres, int n, uint32_t* keys, hipStream_t s) {
-    if (n <= 0) return;
-    k_hash<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, pospres, n, keys);
-}
-
-void launch_hash_table(const RunDesc* d, const int32_t* run_of, const float4* pospres, int n, uint32_t* keys,
-                       hipStream_t s) {
-    if (n <= 0) return;
-    k_hash<DescTable><<<blocks_for(n, 256), 256, 0, s>>>(DescTable{d, run_of}, pospres, n, keys);
-}
-
-void launch_radix_sort(GridBuffers& g, int n, int bits, hipStream_t s) {
-    if (n <= 0) {
-        g.sorted_slot = 0;
-        return;
-    }
+// Radix passes after the fused first count; pass 0 reads hist from
+// k_hash_count.
+static void radix_passes(GridBuffers& g, int n, int bits, hipStream_t s) {
     const int ntiles = rs_tiles(n);
     const int passes = (bits + 7) / 8;
     int src = 0;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
-        k_radix_count<<<ntiles, RS_THREADS, 0, s>>>(g.keys[src], n, shift, g.hist, ntiles);
+        if (p > 0) k_radix_count<<<ntiles, RS_THREADS, 0, s>>>(g.keys[src], n, shift, g.hist, ntiles);
         k_radix_scan<<<RS_BINS, RS_THREADS, 0, s>>>(g.hist, ntiles, g.totals);
         k_radix_scatter<<<ntiles, RS_THREADS, 0, s>>>(g.keys[src], p == 0 ? nullptr : g.vals[src],
                                                        g.keys[1 - src], g.vals[1 - src], n, shift, g.hist,
@@ -314,11 +337,29 @@ void launch_radix_sort(GridBuffers& g, int n, int bits, hipStream_t s) {
     g.sorted_slot = src;
 }
 
-void launch_cell_table(GridBuffers& g, int n, int64_t ncell_total, int zbits, hipStream_t s) {
-    const int fill_blocks = (int)std::min<int64_t>(blocks_for(ncell_total, 256), 4096);
-    if (ncell_total > 0) k_fill2<<<fill_blocks, 256, 0, s>>>(g.cell_beg, g.cell_end, ncell_total, -1);
-    if (n > 0)
-        k_cell_table<<<blocks_for(n, 256), 256, 0, s>>>(g.keys[g.sorted_slot], n, zbits, g.cell_beg, g.cell_end);
+template <class DP>
+static void grid_build(const DP& dp, const float4* pospres, int n, int bits, int64_t ncell_total, int zbits,
+                       GridBuffers& g, hipStream_t s) {
+    if (n <= 0) {
+        g.sorted_slot = 0;
+        const int fill_blocks = (int)std::min<int64_t>(blocks_for(ncell_total, 256), 4096);
+        if (ncell_total > 0) k_fill2<<<fill_blocks, 256, 0, s>>>(g.cell_beg, g.cell_end, ncell_total, -1);
+        return;
+    }
+    const int ntiles = rs_tiles(n);
+    k_hash_count<DP><<<ntiles, RS_THREADS, 0, s>>>(dp, pospres, n, g.keys[0], g.hist, ntiles, g.cell_beg,
+                                                   g.cell_end, ncell_total);
+    radix_passes(g, n, bits, s);
+    k_cell_table<<<blocks_for(n, 256), 256, 0, s>>>(g.keys[g.sorted_slot], n, zbits, g.cell_beg, g.cell_end);
+}
+
+void launch_grid_build_one(const RunDesc& d, const float4* pospres, GridBuffers& g, hipStream_t s) {
+    grid_build(DescOne{d}, pospres, d.np, rs_bits(d.par.ncell) + d.zbits, d.par.ncell, d.zbits, g, s);
+}
+
+void launch_grid_build_table(const RunDesc* d, const int32_t* run_of, const float4* pospres, int n,
+                             int64_t ncell_total, int zbits, GridBuffers& g, hipStream_t s) {
+    grid_build(DescTable{d, run_of}, pospres, n, rs_bits(ncell_total) + zbits, ncell_total, zbits, g, s);
 }
 
 void launch_permute(const uint32_t* perm, int n, const DevState& src, const DevState& dst, const DevIni* ini_src,

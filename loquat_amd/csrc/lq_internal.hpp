@@ -33,8 +33,8 @@ void fill_desc_static(RunDesc& d, const lq_params& par, const lq_vw_params* vw);
 // ---------------------------------------------------------------- grid
 // Radix-sort geometry.
 constexpr int RS_THREADS = 256;
-constexpr int RS_IPT = 16;
-constexpr int RS_TILE = RS_THREADS * RS_IPT;  // 4096 keys per tile
+constexpr int RS_IPT = 8;
+constexpr int RS_TILE = RS_THREADS * RS_IPT;  // 2048 keys per tile
 constexpr int RS_BINS = 256;
 
 inline int rs_tiles(int n) { return (n + RS_TILE - 1) / RS_TILE; }
@@ -60,14 +60,13 @@ struct GridBuffers {
 lq_status alloc_grid(GridBuffers& g, int64_t np_cap, int64_t ncell_cap);
 void free_grid(GridBuffers& g);
 
-// Cell hash of N particles (one run by value, or a table).
-void launch_hash_one(const RunDesc& d, const float4* pospres, int n, uint32_t* keys, hipStream_t s);
-void launch_hash_table(const RunDesc* d, const int32_t* run_of, const float4* pospres, int n, uint32_t* keys,
-                       hipStream_t s);
-// Stable LSD radix sort of keys[0] with values = iota; result in slot g.sorted_slot.
-void launch_radix_sort(GridBuffers& g, int n, int bits, hipStream_t s);
-// cell_beg / cell_end of the sorted keys (-1 for empty cells).
-void launch_cell_table(GridBuffers& g, int n, int64_t ncell_total, int zbits, hipStream_t s);
+// Neighbour grid of N particles (one run by value, or a table of runs with
+// zbits z-bin bits in the key): cell keys, stable LSD radix sort of
+// (key, index) over ceil(bits/8) 8-bit passes (result in slot
+// g.sorted_slot), and cell_beg / cell_end (-1 for empty cells).
+void launch_grid_build_one(const RunDesc& d, const float4* pospres, GridBuffers& g, hipStream_t s);
+void launch_grid_build_table(const RunDesc* d, const int32_t* run_of, const float4* pospres, int n,
+                             int64_t ncell_total, int zbits, GridBuffers& g, hipStream_t s);
 // dst[k] = src[perm[k]] for all state arrays (and ini when both set).  When
 // str_dst is non-null the permuted stress goes there instead of dst.str.
 void launch_permute(const uint32_t* perm, int n, const DevState& src, const DevState& dst, const DevIni* ini_src,

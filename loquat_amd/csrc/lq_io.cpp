@@ -3,11 +3,14 @@
 //   lq_params_derive    InitializeParameters   src/loquat.cu:111-139 (+ dt, :331-333)
 //   lq_load_parameters  LoquatIo::LoadParameters src/loquat_io.cu:69-201
 //   lq_load_particles   LoquatIo::LoadParticles  src/loquat_io.cu:204-237
+//   lq_save_output      LoquatIo::SaveXml + SaveParticles  src/loquat_io.cu:240-549
+//   lq_load_restart     (FP64 positions and void ratio of an output; no counterpart)
 //
 // The XML is the reference's flat list of <key type="int|float">value</key>
 // elements under one root; a small reader replaces tinyxml2 + Boost.Variant.
 // Errors come back as status codes with a message (lq_last_error) instead of
 // exit(EXIT_FAILURE).
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -335,5 +338,244 @@ extern "C" lq_status lq_load_particles(const char* prefix, const lq_params* par,
         for (int t = 0; t < 6; ++t) s[t] = v[10 + t];
         for (int t = 0; t < 6; ++t) n[t] = v[16 + t];
     }
+    return LQ_OK;
+}
+
+// ------------------------------------------------------------------ output
+// LoquatIo::SaveXml / SaveParticles (src/loquat_io.cu:240-549), written so
+// that every output is a restart: see lq_save_output in the header for the
+// differences from the reference.
+namespace {
+
+// Shortest form that reads back as the same float (9 significant digits).
+struct F9 {
+    float v;
+};
+std::ostream& operator<<(std::ostream& o, F9 x) {
+    char b[32];
+    std::snprintf(b, sizeof(b), "%.9g", (double)x.v);
+    return o << b;
+}
+
+bool write_xml(const std::string& name, const lq_params& p, const lq_vw_params* vw) {
+    std::ofstream o(name.c_str(), std::ofstream::out);
+    if (!o) return false;
+    auto fl = [&](const char* k, float v) { o << "  <" << k << " type=\"float\">" << F9{v} << "</" << k << ">\n"; };
+    auto in = [&](const char* k, int v) { o << "  <" << k << " type=\"int\">" << v << "</" << k << ">\n"; };
+    o << "<?xml version=\"1.0\" encoding=\"utf-8\" ?>\n<parameters>\n";
+    in("number-of-particles", p.np);
+    in("number-of-boundary-particles", p.npb);
+    in("number-of-material-particles", p.npm);
+    fl("domain-size-x", p.domain_size_x);
+    fl("domain-size-y", p.domain_size_y);
+    fl("domain-size-z", p.domain_size_z);
+    fl("domain-min-x", p.domain_min_x);
+    fl("domain-min-y", p.domain_min_y);
+    fl("domain-min-z", p.domain_min_z);
+    fl("particle-size", p.dr);
+    fl("h-over-dr", p.hdr);
+    fl("reference-density", p.rho0);
+    fl("speed-of-sound", p.cs);
+    fl("cfl-number", p.cfl);
+    in("constitutive-model", p.constitutive_model);
+    if (p.constitutive_model == 1) {
+        fl("elastic-K", p.model_par_1);
+        fl("elastic-G", p.model_par_2);  // the reader's key (the reference writes poisson-G)
+        fl("DP-kphi", p.model_par_3);
+        fl("DP-kpsi", p.model_par_4);
+        fl("DP-kcohesion", p.model_par_5);
+    } else if (p.constitutive_model == 2) {
+        fl("hypo-par-c1", p.model_par_1);
+        fl("hypo-par-c2", p.model_par_2);
+        fl("hypo-par-c3", p.model_par_3);
+        fl("hypo-par-c4", p.model_par_4);
+        fl("cohesion", p.model_par_5);
+    } else if (p.constitutive_model == 3 && vw) {
+        fl("vw-phi-c", vw->phi_c);
+        fl("vw-hs", vw->hs);
+        fl("vw-n", vw->n);
+        fl("vw-ed0", vw->ed0);
+        fl("vw-ec0", vw->ec0);
+        fl("vw-ei0", vw->ei0);
+        fl("vw-alpha", vw->alpha);
+        fl("vw-beta", vw->beta);
+    }
+    fl("acceleration-x", p.acc_x);
+    fl("acceleration-y", p.acc_y);
+    fl("acceleration-z", p.acc_z);
+    fl("time-max", p.time_max);
+    fl("time", p.time);
+    fl("time-step", p.dt);
+    fl("output-frequency", p.output_frequency);
+    in("output-number", p.output_number);
+    in("density-shepard-filter", p.density_shepard_filter);
+    in("stress-shepard-filter", p.stress_shepard_filter);
+    in("dynamic-time-step", p.dynamic_time_step);
+    in("integration-method", p.integration_method);
+    in("viscosity-type", p.viscosity_type);
+    if (p.viscosity_type == 1) fl("artificial-viscosity-alpha", p.artificial_viscosity_alpha);
+    in("artificial-pressure", p.artificial_pressure);
+    if (p.artificial_pressure == 1) fl("artificial-pressure-coefficient", p.artificial_pressure_coefficient);
+    in("output-format", p.output_format);
+    o << "</parameters>\n";
+    return (bool)o;
+}
+
+bool write_dat(const std::string& name, int np, const lq_host_particles& h) {
+    std::ofstream o(name.c_str(), std::ofstream::out);
+    if (!o) return false;
+    for (int i = 0; i < np; ++i) {
+        const float* pp = h.pospres + 4 * (size_t)i;
+        const float* vr = h.velrhop + 4 * (size_t)i;
+        const float* id = h.idep + 4 * (size_t)i;
+        const float* s = h.str + 6 * (size_t)i;
+        const float* n = h.stn + 6 * (size_t)i;
+        o << F9{pp[0]} << ' ' << F9{pp[1]} << ' ' << F9{pp[2]} << ' ' << F9{pp[3]} << ' ' << F9{vr[0]} << ' '
+          << F9{vr[1]} << ' ' << F9{vr[2]} << ' ' << F9{vr[3]} << ' ' << F9{id[0]} << ' ' << F9{id[1]};
+        for (int t = 0; t < 6; ++t) o << ' ' << F9{s[t]};
+        for (int t = 0; t < 6; ++t) o << ' ' << F9{n[t]};
+        o << '\n';
+    }
+    return (bool)o;
+}
+
+// Binary sidecar with what the 22-column .dat cannot hold: FP64 positions
+// and the void ratio (idep.z) of constitutive model 3.
+const char kRstMagic[8] = {'L', 'Q', 'R', 'S', 'T', '0', '0', '1'};
+
+bool write_rst(const std::string& name, int np, const lq_host_particles& h, const double* posd) {
+    FILE* f = std::fopen(name.c_str(), "wb");
+    if (!f) return false;
+    const int64_t n = np;
+    bool ok = std::fwrite(kRstMagic, 1, 8, f) == 8 && std::fwrite(&n, 8, 1, f) == 1;
+    std::vector<double> xyz(3 * (size_t)np);
+    for (int i = 0; i < np; ++i)
+        for (int a = 0; a < 3; ++a)
+            xyz[3 * (size_t)i + a] = posd ? posd[3 * (size_t)i + a] : (double)h.pospres[4 * (size_t)i + a];
+    std::vector<float> e((size_t)np);
+    for (int i = 0; i < np; ++i) e[i] = h.idep[4 * (size_t)i + 2];
+    ok = ok && (np == 0 || std::fwrite(xyz.data(), 8, xyz.size(), f) == xyz.size());
+    ok = ok && (np == 0 || std::fwrite(e.data(), 4, e.size(), f) == e.size());
+    ok = (std::fclose(f) == 0) && ok;
+    return ok;
+}
+
+// One .vtu per object id (idep.x), objects in descending id order and
+// particles in input order inside each (src/loquat_io.cu:373-545).
+bool write_vtu(const std::string& dir, int number, int np, const lq_host_particles& h) {
+    std::vector<int> idx((size_t)np);
+    for (int i = 0; i < np; ++i) idx[i] = i;
+    auto id_of = [&](int i) { return (int)h.idep[4 * (size_t)i]; };
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return id_of(a) > id_of(b); });
+    size_t s0 = 0;
+    while (s0 < idx.size()) {
+        const int id = id_of(idx[s0]);
+        size_t s1 = s0;
+        while (s1 < idx.size() && id_of(idx[s1]) == id) ++s1;
+        const size_t cnt = s1 - s0;
+        std::ostringstream nm;
+        nm << dir << "/" << id << "_" << number << ".vtu";
+        std::ofstream o(nm.str().c_str(), std::ofstream::out);
+        if (!o) return false;
+        auto arr = [&](const char* name, int comps, auto&& put) {
+            o << "        <DataArray type=\"Float32\" Name=\"" << name << "\"";
+            if (comps > 1) o << " NumberOfComponents=\"" << comps << "\"";
+            o << " format=\"ascii\">\n";
+            for (size_t r = s0; r < s1; ++r) put(idx[r]);
+            o << "</DataArray>\n";
+        };
+        o << "<?xml version=\"1.0\"?>\n"
+             "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n"
+             "  <UnstructuredGrid GhostLevel=\"0\">\n"
+          << "    <Piece NumberOfPoints=\"" << cnt << "\" NumberOfCells=\"" << cnt << "\" Name=\"" << id << "\">\n"
+          << "      <PointData>\n";
+        arr("Pressure", 1, [&](int i) { o << F9{h.pospres[4 * (size_t)i + 3]} << ' '; });
+        arr("Velocity", 3, [&](int i) {
+            for (int a = 0; a < 3; ++a) o << F9{h.velrhop[4 * (size_t)i + a]} << ' ';
+        });
+        arr("Density", 1, [&](int i) { o << F9{h.velrhop[4 * (size_t)i + 3]} << ' '; });
+        arr("PlasticStrain", 1, [&](int i) { o << F9{h.idep[4 * (size_t)i + 1]} << ' '; });
+        arr("Stress", 6, [&](int i) {
+            for (int a = 0; a < 6; ++a) o << F9{h.str[6 * (size_t)i + a]} << ' ';
+        });
+        arr("Strain", 6, [&](int i) {
+            for (int a = 0; a < 6; ++a) o << F9{h.stn[6 * (size_t)i + a]} << ' ';
+        });
+        o << "      </PointData>\n      <CellData>\n      </CellData>\n      <Points>\n";
+        arr("Points", 3, [&](int i) {
+            for (int a = 0; a < 3; ++a) o << F9{h.pospres[4 * (size_t)i + a]} << ' ';
+        });
+        o << "      </Points>\n      <Cells>\n"
+             "        <DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n";
+        for (size_t j = 0; j < cnt; ++j) o << j << ' ';
+        o << "</DataArray>\n        <DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n";
+        for (size_t j = 0; j < cnt; ++j) o << j + 1 << ' ';
+        o << "</DataArray>\n        <DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n";
+        for (size_t j = 0; j < cnt; ++j) o << "1 ";
+        o << "</DataArray>\n      </Cells>\n    </Piece>\n  </UnstructuredGrid>\n</VTKFile>\n";
+        if (!o) return false;
+        s0 = s1;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" lq_status lq_save_output(const char* dir, const lq_params* par, const lq_vw_params* vw,
+                                    const lq_host_particles* h, const double* posd) {
+    if (!dir || !par || !h || !h->pospres || !h->velrhop || !h->idep || !h->str || !h->stn) {
+        lq::set_error("lq_save_output: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (par->np < 0) {
+        lq::set_error("lq_save_output: negative particle count");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (par->constitutive_model == 3 && !vw) {
+        lq::set_error("lq_save_output: constitutive model 3 needs lq_vw_params");
+        return LQ_ERR_INVALID_ARG;
+    }
+    const std::string d(dir);
+    const std::string n = std::to_string(par->output_number);
+    if (!write_xml(d + "/output_" + n + ".xml", *par, vw)) {
+        lq::set_error("Cannot write " + d + "/output_" + n + ".xml");
+        return LQ_ERR_IO;
+    }
+    if (!write_dat(d + "/" + n + ".dat", par->np, *h) || !write_rst(d + "/" + n + ".rst", par->np, *h, posd)) {
+        lq::set_error("Cannot write " + d + "/" + n + ".dat / .rst");
+        return LQ_ERR_IO;
+    }
+    if (par->output_format == 2 && !write_vtu(d, par->output_number, par->np, *h)) {
+        lq::set_error("Cannot write the .vtu files in " + d);
+        return LQ_ERR_IO;
+    }
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_load_restart(const char* prefix, const lq_params* par, double* posd, float* idep) {
+    if (!prefix || !par || !posd) {
+        lq::set_error("lq_load_restart: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    const std::string name = std::string(prefix) + ".rst";
+    FILE* f = std::fopen(name.c_str(), "rb");
+    if (!f) {
+        lq::set_error("Cannot open " + name);
+        return LQ_ERR_IO;
+    }
+    char magic[8];
+    int64_t n = -1;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kRstMagic, 8) == 0 &&
+              std::fread(&n, 8, 1, f) == 1 && n == par->np;
+    std::vector<float> e((size_t)std::max<int64_t>(n, 0));
+    ok = ok && (n == 0 || std::fread(posd, 8, 3 * (size_t)n, f) == 3 * (size_t)n);
+    ok = ok && (n == 0 || std::fread(e.data(), 4, e.size(), f) == e.size());
+    std::fclose(f);
+    if (!ok) {
+        lq::set_error(name + ": not a restart file for " + std::to_string(par->np) + " particles");
+        return LQ_ERR_PARSE;
+    }
+    if (idep)
+        for (int64_t i = 0; i < n; ++i) idep[4 * (size_t)i + 2] = e[i];
     return LQ_OK;
 }

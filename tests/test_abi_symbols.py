@@ -42,5 +42,5 @@ def test_version_and_error_without_gpu():
     # argument validation runs on the host and reports through lq_last_error
     assert lib.lq_engine_create(0, 0, 0, None) == 1
     assert len(lib.lq_last_error()) > 0
-    assert lib.lq_set_interaction_variant(7) == 1
+    assert lib.lq_set_interaction_variant(abi.INTERACTION_VARIANTS) == 1
     assert lib.lq_get_interaction_variant() in range(abi.INTERACTION_VARIANTS)

@@ -121,11 +121,11 @@ def test_interaction_variants_agree_in_engine(gpu):
         e.step(5)
         res.append(e.download(rid))
         e.close()
-    abi.check(abi.lib().lq_set_interaction_variant(3))
+    abi.check(abi.lib().lq_set_interaction_variant(abi.DEFAULT_INTERACTION_VARIANT))
     for r in res[1:]:
         for k in ("posd", "velrhop", "str"):
             assert rel_err(res[0][k], r[k]) < 1e-5
-    # variants 3 to 6 evaluate the same pairs in the same order
+    # variants 3 to 6 run the same pair code on the same pairs in the same order
     for r in res[2:]:
         assert _same(res[1], r)
 

@@ -147,7 +147,7 @@ def test_adami_parity(gpu):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", list(range(abi.INTERACTION_VARIANTS)))
 @pytest.mark.parametrize("which", ["small3d", "c1"])
 def test_interaction_parity(gpu, variant, which):
     dev = _dev()
@@ -164,7 +164,7 @@ def test_interaction_parity(gpu, variant, which):
         h = rt.host()
         nb = nbr.cpu().numpy()
     finally:
-        abi.check(abi.lib().lq_set_interaction_variant(3))
+        abi.check(abi.lib().lq_set_interaction_variant(abi.DEFAULT_INTERACTION_VARIANT))
     npb = p.npb
     assert np.array_equal(nb[npb:], onbr[npb:])
     assert nb[npb:].min() > 0
