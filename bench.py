@@ -31,6 +31,7 @@ FLOPS_PER_PAIR = 80            # algorithmic FP32 flops per neighbour pair
 BYTES_READ_PER_PARTICLE = 64   # x (FP64, 24) + v (12) + rho (4) + stress (24)
 BYTES_WRITE_PER_MATERIAL = 52  # a (12) + drho (4) + spin (12) + strain rate (24)
 BYTES_PER_CELL = 8             # cell_beg + cell_end
+WORKLOAD_ID = "column_3d dx=5mm npm=1024000"  # the key of this workload in profiles/*.json
 KERNEL_NAMES = {0: "k_interaction_direct", 1: "k_interaction_lds", 2: "k_interaction_unrolled",
                 3: "k_interaction_v3<DescTable, 1, false, 1, 4>", 4: "k_interaction_v3<DescTable, 2, false, 1, 4>",
                 5: "k_interaction_v3<DescTable, 1, true, 1, 4>", 6: "k_interaction_v3<DescTable, 2, true, 1, 4>"}
@@ -41,17 +42,24 @@ def log(msg: str) -> None:
 
 
 def cpu_baseline(case, steps: int) -> dict:
-    """FP64 oracle (C++/OpenMP) on this host: initial sort, 1 warm-up step,
-    then `steps` timed steps of the same workload."""
+    """FP64 oracle (C++/OpenMP) on this host: initial sort, 2 warm-up steps,
+    then `steps` timed steps of the same workload on all threads, and one
+    more step on one thread."""
     from oracle import lq_oracle as O
 
     p = case.derived()
     st = O.OState.from_case(case)
     O.initial_sort(p, st)
-    O.run(p, case.vw, st, 1, 0)
+    O.run(p, case.vw, st, 2, 0)
     t0 = time.perf_counter()
-    O.run(p, case.vw, st, steps, 1)
+    O.run(p, case.vw, st, steps, 2)
     dt = time.perf_counter() - t0
+    threads = O.num_threads()
+    O.set_num_threads(1)
+    t1 = time.perf_counter()
+    O.run(p, case.vw, st, 1, 2 + steps)
+    dt1 = time.perf_counter() - t1
+    O.set_num_threads(threads)
     cpu_model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -64,12 +72,31 @@ def cpu_baseline(case, steps: int) -> dict:
     return {
         "value": case.npm * steps / dt / 1e6,
         "unit": "M particle-steps/s",
-        "cores": O.num_threads(),
+        "cores": threads,
         "kind": "port",
-        "sample": f"FP64 oracle, same 1.55 M-particle column: 1 warm-up + {steps} timed full steps "
-                  f"({dt:.1f} s), OpenMP threads={O.num_threads()}, {cpu_model}",
+        "sample": f"FP64 oracle, same 1.55 M-particle column: 2 warm-up + {steps} timed full steps "
+                  f"({dt:.1f} s), OpenMP threads={threads}, {cpu_model}",
         "s_per_step": dt / steps,
+        "one_thread_s_per_step": dt1,
     }
+
+
+def measured_traffic(kernel: str) -> tuple[float | None, str | None]:
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 summary of the same command (FETCH_SIZE + WRITE_SIZE per
+    dispatch, each scaled by what the counter reads for a device copy of
+    known size), or None when no summary covers this kernel."""
+    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+    if not os.path.isdir(d):
+        return None, None
+    for f in sorted(os.listdir(d)):
+        if not f.endswith(".json"):
+            continue
+        with open(os.path.join(d, f)) as fh:
+            j = json.load(fh)
+        if j.get("kernel") == kernel and j.get("workload") == WORKLOAD_ID:
+            return float(j["traffic_bytes"]), "profiles/" + f
+    return None, None
 
 
 def pair_count(case) -> int:
@@ -271,7 +298,7 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=None, help="interaction kernel variant (without --autotune)")
     ap.add_argument("--autotune", type=int, default=1, help="1: time every interaction variant on the "
@@ -350,6 +377,7 @@ def main() -> None:
     bytes_launch = case.np * BYTES_READ_PER_PARTICLE + case.npm * BYTES_WRITE_PER_MATERIAL + p.ncell * BYTES_PER_CELL
     flops_launch = pairs * FLOPS_PER_PAIR
     gbs = bytes_launch / (inter_launch_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(KERNEL_NAMES[variant])
     tflops = flops_launch / (inter_launch_ms * 1e-3) / 1e12
 
     value = case.npm * a.steps / (gpu_ms * 1e-3) / 1e6
@@ -379,12 +407,14 @@ def main() -> None:
         },
         "roofline": {
             "kernel": KERNEL_NAMES[variant],
+            "measured_by": "HIP events around the interaction launches on the engine stream (lq_engine_profile)",
             "bound": "hbm",
             "achieved": round(gbs, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "launch_ms": round(inter_launch_ms, 5),
             "bytes_per_launch": bytes_launch,
             "valu": {
@@ -423,7 +453,7 @@ def main() -> None:
         log(f"slope: {out['slope_vw']} ({time.perf_counter() - t:.1f} s)")
     if not a.no_cpu:
         log(f"CPU baseline: FP64 oracle, {a.cpu_steps} steps ...")
-        out["cpu_baseline"] = cpu_baseline(cases.column_3d(nsteps=a.cpu_steps + 1), a.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(cases.column_3d(nsteps=a.cpu_steps + 3), a.cpu_steps)
     print(json.dumps(out), flush=True)
 
 

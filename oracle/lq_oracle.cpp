@@ -642,6 +642,14 @@ void lqo_constitutive(const lq_params* par, const lq_vw_params* vw, double dt, c
     s6[5] = s.yz;
 }
 
+void lqo_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 int lqo_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -55,6 +55,7 @@ def lib() -> C.CDLL:
         L.lqo_initial_sort.argtypes = [vp, C.POINTER(_State)]
         L.lqo_run.argtypes = [vp, vp, C.POINTER(_State), C.c_int, C.c_int]
         L.lqo_num_threads.restype = C.c_int
+        L.lqo_set_num_threads.argtypes = [C.c_int]
         L.lqo_constitutive.argtypes = [vp, vp, C.c_double, dp, dp, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
@@ -259,6 +260,10 @@ def run(par, vw, st: OState, nsteps: int, step0: int = 0) -> None:
 
 def num_threads() -> int:
     return lib().lqo_num_threads()
+
+
+def set_num_threads(n: int) -> None:
+    lib().lqo_set_num_threads(int(n))
 
 
 def unpermute(st: OState) -> OState:

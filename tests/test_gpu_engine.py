@@ -225,3 +225,42 @@ def test_cell_order_changes_only_sum_order(gpu):
     assert rel_err(res[0]["posd"] - x0, res[1]["posd"] - x0) < 1e-5
     for k in ("velrhop", "str"):
         assert rel_err(res[0][k], res[1][k]) < 1e-5, k
+
+
+@pytest.mark.gpu
+def test_restart_from_output_continues_the_run(gpu, tmp_path):
+    """Run 8 steps, write an output, read it back as a new project, run 8
+    more: the reloaded state equals the written one bit for bit, and the
+    continued run matches the uninterrupted 16-step run (the restart re-sorts
+    from input order, so only FP32 sum order differs).  No Shepard filter: its
+    cadence counts steps from the start of each launch, as the reference's
+    step counter does (src/loquat.cu:307,401), so a restart shifts it."""
+    case = cases.perturb(cases.small_column_3d(seed=81), seed=82)
+    whole, (rid,) = _engine([case], 16)
+    whole.step(16)
+    ref = whole.download(rid)
+    whole.close()
+
+    e, (rid,) = _engine([case], 16)
+    e.step(8)
+    mid = e.download(rid)
+    e.close()
+    par = case.par.copy()
+    par.output_number = 1
+    abi.save_output(str(tmp_path), par, case.vw, mid, mid["posd"])
+    p2, vw2 = abi.load_parameters(str(tmp_path / "output_1"))
+    arrs = abi.load_particles(str(tmp_path / "1"), p2)
+    posd = abi.load_restart(str(tmp_path / "1"), p2, arrs["idep"])
+    for k in ("pospres", "velrhop", "str", "stn"):
+        assert np.array_equal(arrs[k], mid[k]), k
+    assert np.array_equal(posd, mid["posd"])
+    resumed = cases.Case("resumed", p2, vw2, arrs["pospres"], arrs["velrhop"], arrs["idep"], arrs["str"],
+                         arrs["stn"], posd, 8)
+    e, (rid,) = _engine([resumed], 8)
+    e.step(8)
+    got = e.download(rid)
+    e.close()
+    x0 = case.posd
+    assert rel_err(got["posd"] - x0, ref["posd"] - x0) < 1e-5
+    for k in ("velrhop", "str"):
+        assert rel_err(got[k], ref[k]) < 1e-5, k
