@@ -249,6 +249,17 @@ typedef struct lq_run_input {
     int64_t nsteps;              /* steps to run; <= 0: until time >= time_max (src/loquat.cu:322) */
 } lq_run_input;
 
+/* Time step.  par.dynamic_time_step = 0 (the reference's only working mode,
+ * src/loquat.cu:331-333): dt = cfl*h/cs every step.  = 1: each step's dt is
+ * chosen on the device before the step as
+ *     dt = cfl * min(h / (cs + max|v|), sqrt(h / max|a|))
+ * over the run's material particles (a = the previous step's corrector
+ * rates; no |a| term on the run's first step).  The reference parses the flag
+ * but its call is commented out (src/loquat.cu:325-327) and its body is not
+ * shipped, so this rule is a specification decision.  A dynamic run added
+ * with nsteps <= 0 ends on the device once time >= time_max;
+ * lq_engine_run_info reports the device's step count and time. */
+
 /* Capacities bound the packed layout: particles, cells (sum over runs), runs. */
 lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t max_runs, lq_engine** out);
 lq_status lq_engine_destroy(lq_engine* eng);
@@ -267,13 +278,27 @@ lq_status lq_engine_remove_run(lq_engine* eng, int32_t run_id);
 lq_status lq_engine_step(lq_engine* eng, int32_t nsteps);
 lq_status lq_engine_synchronize(lq_engine* eng);
 
-/* Host-side progress of one run: steps done, FP64 time, finished flag. */
+/* Progress of one run as the device has it: steps done, FP64 time, finished
+ * flag, and np = the number of particles the run was added with (the row
+ * count lq_engine_download writes).  Synchronizes the engine stream. */
 lq_status lq_engine_run_info(const lq_engine* eng, int32_t run_id, int64_t* steps_done,
                              double* time, int32_t* finished, int32_t* np);
 
 /* Copies one run's state back in input-file order (reference layout host
- * arrays, np rows; posd double[3*np] optional).  Synchronous. */
+ * arrays, np rows as run_info reports; posd double[3*np] optional).  Rows of
+ * particles removed by lq_engine_compact get NaN positions (pospres xyz and
+ * posd) and are otherwise left as they were.  Synchronous. */
 lq_status lq_engine_download(lq_engine* eng, int32_t run_id, lq_host_particles* out, double* posd);
+
+/* Particle deletion (Parameters::np_delete / ncell_delete, include/types.h:93,108,
+ * and the LoquatGrid::ResizeArrayLength hook, src/loquat_grid.h:58, which the
+ * reference never calls): removes from every run the material particles that
+ * lie outside its cell grid (cell ncell - 1, where the reference's index rule
+ * sends them and where no neighbour search finds them), shrinks np and npm,
+ * adds them to par.np_delete, and packs the runs again.  The surviving
+ * particles' states and order are unchanged, so their results are the same
+ * as without compaction.  removed (nullable): particles removed in total. */
+lq_status lq_engine_compact(lq_engine* eng, int64_t* removed);
 
 /* Collects the per-run failure flags the corrector raises on the device when
  * a particle's new state is not finite (NaN/Inf).  A failed run stops

@@ -141,6 +141,7 @@ _PROTOS = {
     "lq_engine_neighbour_counts": (i32, [vp, i32, vp]),
     "lq_engine_profile": (i32, [vp, i32, C.POINTER(f64)]),
     "lq_engine_poll": (i32, [vp, C.POINTER(i32)]),
+    "lq_engine_compact": (i32, [vp, C.POINTER(C.c_int64)]),
     "lq_engine_autotune": (i32, [vp, i32, C.POINTER(i32), C.POINTER(f64)]),
     "lq_engine_run_failed": (i32, [vp, i32, C.POINTER(i32)]),
     "lq_server_create": (i32, [C.POINTER(LqServerConfig), C.POINTER(vp)]),

@@ -34,6 +34,7 @@ struct RunDesc {
     int32_t step;         // steps completed
     int32_t nsteps;       // steps requested
     int32_t zbits;        // sort key = cell << zbits | z-bin in cell (0: cell only)
+    int32_t until_time;   // 1: the run ends when time >= time_max (dynamic dt), else after nsteps
     double time;          // FP64 simulated time (src/loquat.cu:305,455)
     lq_vw_params vw;      // constitutive model 3 constants
     float vw_a, vw_fb0;   // derived model-3 constants (a, and the f_b prefactor)

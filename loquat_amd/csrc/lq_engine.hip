@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <limits>
 #include <vector>
 
 #include "lq_internal.hpp"
@@ -312,7 +313,8 @@ struct RunSlot {
     int32_t id;
     RunDesc desc;  // host mirror; dynamic fields advanced in lockstep with k_step_end
     int64_t nsteps;
-    int32_t failed;  // non-finite state seen by the corrector (lq_engine_poll)
+    int32_t failed;    // non-finite state seen by the corrector (lq_engine_poll)
+    int32_t np_input;  // particles the run was added with (deleted ones included)
 };
 
 struct Buffers {
@@ -323,6 +325,7 @@ struct Buffers {
     int32_t* run_of;
     RunDesc* desc;
     int32_t* fail;  // per-run flag, raised on the device by the corrector
+    uint32_t* dtred;  // per-run |v|max, |a|max of the dynamic time step (2 per run, kept zero between steps)
     int32_t* mat_idx;   // particle index of the t-th material particle (all runs)
     int32_t* wall_idx;  // particle index of the t-th wall particle (all runs)
     GridBuffers grid;
@@ -353,6 +356,7 @@ struct lq_engine {
     bool use_graphs = true;
     int variant = -1;  // interaction variant pinned by lq_engine_autotune (-1: library default)
     int zbits = 4;     // z-bins per cell (log2) in the engine's sort key
+    bool any_dynamic = false;  // some resident run has dynamic-time-step = 1
     hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 };
 
@@ -409,6 +413,8 @@ void enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
     const int nr = (int)e->runs.size();
     const int var = e->variant >= 0 ? e->variant : g_variant;
     launch_step_begin(b.desc, nr, s);
+    if (e->any_dynamic)
+        launch_dynamic_dt(b.desc, b.run_of, b.S[cur], b.R, IndexMap{b.mat_idx, 0}, (int)e->nmat_total, nr, b.dtred, s);
     int c = cur;
     for (int half = 0; half < 2; ++half) {
         const bool corr = half == 1;
@@ -473,14 +479,36 @@ void host_step_end(lq_engine* e) {
 
 // Uploads the run table, the failure flags and the wall / material index maps
 // of the current layout.
+// Refreshes the host mirror of what the device advances by itself (step
+// counter, time, dt, active flag): required before the mirror is uploaded
+// again, and for reporting runs whose dt the device chooses.
+lq_status sync_descs(lq_engine* e) {
+    const size_t nr = e->runs.size();
+    if (nr == 0) return LQ_OK;
+    std::vector<RunDesc> h(nr);
+    LQ_HIP_TRY(hipStreamSynchronize(e->stream));
+    LQ_HIP_TRY(hipMemcpy(h.data(), e->b.desc, nr * sizeof(RunDesc), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nr; ++i) {
+        RunDesc& d = e->runs[i].desc;
+        d.step = h[i].step;
+        d.time = h[i].time;
+        d.par.dt = h[i].par.dt;
+        d.active = h[i].active;
+        d.shepard_due = h[i].shepard_due;
+    }
+    return LQ_OK;
+}
+
 lq_status upload_descs(lq_engine* e) {
     std::vector<RunDesc> h(e->runs.size());
     std::vector<int32_t> f(e->runs.size());
     std::vector<int32_t> walls, mats;
     const int zb = zbits_eff(e);
+    e->any_dynamic = false;
     for (size_t i = 0; i < e->runs.size(); ++i) {
         RunDesc& d = e->runs[i].desc;
         d.zbits = zb;
+        e->any_dynamic = e->any_dynamic || d.par.dynamic_time_step == 1;
         h[i] = d;
         f[i] = e->runs[i].failed;
         for (int k = 0; k < d.npb; ++k) walls.push_back(d.p_off + k);
@@ -588,7 +616,8 @@ extern "C" lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t
          dmalloc(b.R.acestn, np_cap, e->owned) && dmalloc(b.T, np_cap, e->owned) &&
          dmalloc(b.run_of, np_cap, e->owned) && dmalloc(b.desc, max_runs, e->owned) &&
          dmalloc(b.fail, max_runs, e->owned) && dmalloc(b.mat_idx, np_cap, e->owned) &&
-         dmalloc(b.wall_idx, np_cap, e->owned);
+         dmalloc(b.wall_idx, np_cap, e->owned) && dmalloc(b.dtred, 2 * (int64_t)max_runs, e->owned);
+    ok = ok && hipMemset(b.dtred, 0, 2 * (size_t)max_runs * sizeof(uint32_t)) == hipSuccess;
     if (!ok || alloc_grid(b.grid, np_cap, ncell_cap) != LQ_OK) {
         for (void* p : e->owned) (void)hipFree(p);
         (void)hipStreamDestroy(e->stream);
@@ -627,7 +656,7 @@ extern "C" lq_status lq_engine_set_cell_order(lq_engine* e, int32_t zbin_bits) {
         set_error("lq_engine_set_cell_order: zbin_bits must be 0..8");
         return LQ_ERR_INVALID_ARG;
     }
-    LQ_HIP_TRY(hipStreamSynchronize(e->stream));
+    LQ_TRY(sync_descs(e));
     e->zbits = zbin_bits;
     LQ_TRY(upload_descs(e));
     drop_graphs(e);
@@ -652,6 +681,7 @@ extern "C" lq_status lq_engine_add_run(lq_engine* e, const lq_run_input* in, int
     lq_params par = in->par;
     LQ_TRY(lq_params_derive(&par));
     LQ_TRY(check_params(&par));
+    LQ_TRY(sync_descs(e));
     if (par.constitutive_model == 3 && !(in->vw.phi_c > 0.0f && in->vw.hs > 0.0f && in->vw.n > 0.0f)) {
         set_error("constitutive model 3 needs phi_c, hs, n > 0");
         return LQ_ERR_INVALID_ARG;
@@ -702,10 +732,19 @@ extern "C" lq_status lq_engine_add_run(lq_engine* e, const lq_run_input* in, int
     fill_desc_static(slot.desc, par, &in->vw);
     slot.desc.p_off = (int32_t)e->np_total;
     slot.desc.cell_base = (int32_t)e->ncell_total;
-    slot.nsteps = in->nsteps > 0 ? in->nsteps : steps_until((double)par.time, (double)par.time_max, (double)par.dt);
+    if (in->nsteps > 0) {
+        slot.nsteps = in->nsteps;
+    } else if (par.dynamic_time_step == 1) {
+        // dt changes every step: the device ends the run at time_max
+        slot.nsteps = 0x7fffffff;
+        slot.desc.until_time = 1;
+    } else {
+        slot.nsteps = steps_until((double)par.time, (double)par.time_max, (double)par.dt);
+    }
     slot.desc.nsteps = (int32_t)std::min<int64_t>(slot.nsteps, 0x7fffffff);
-    slot.desc.active = slot.desc.nsteps > 0 ? 1 : 0;
+    slot.desc.active = slot.desc.nsteps > 0 && (!slot.desc.until_time || par.time < par.time_max) ? 1 : 0;
     slot.failed = 0;
+    slot.np_input = par.np;
     const DevState& S = e->b.S[e->cur];
     const int64_t o = e->np_total;
     std::vector<int32_t> orig(np), run_of(np, (int32_t)e->runs.size());
@@ -757,6 +796,7 @@ extern "C" lq_status lq_engine_remove_run(lq_engine* e, int32_t run_id) {
         set_error("unknown run id");
         return LQ_ERR_INVALID_ARG;
     }
+    LQ_TRY(sync_descs(e));
     e->runs.erase(e->runs.begin() + i);
     return relayout(e);
 }
@@ -942,11 +982,13 @@ extern "C" lq_status lq_engine_run_info(const lq_engine* e, int32_t run_id, int6
         set_error("unknown run id");
         return LQ_ERR_INVALID_ARG;
     }
+    // the device advances step, time and dt on its own: report its values
+    LQ_TRY(sync_descs(const_cast<lq_engine*>(e)));
     const RunDesc& d = e->runs[i].desc;
     if (steps_done) *steps_done = d.step;
     if (time) *time = d.time;
     if (finished) *finished = d.active ? 0 : 1;
-    if (np) *np = d.np;
+    if (np) *np = e->runs[i].np_input;
     return LQ_OK;
 }
 
@@ -1029,12 +1071,15 @@ extern "C" lq_status lq_engine_download(lq_engine* e, int32_t run_id, lq_host_pa
         LQ_HIP_TRY(hipMemcpy(stn.data(), S.stn + o, np * 24, hipMemcpyDeviceToHost));
         LQ_HIP_TRY(hipMemcpy(orig.data(), S.orig + o, np * 4, hipMemcpyDeviceToHost));
     }
+    const int n_in = e->runs[i].np_input;
+    std::vector<char> seen((size_t)n_in, 0);
     for (int k = 0; k < np; ++k) {
         const int t = orig[k];
-        if (t < 0 || t >= np) {
+        if (t < 0 || t >= n_in) {
             set_error("corrupt original-index array");
             return LQ_ERR_STATE;
         }
+        seen[t] = 1;
         if (out->pospres) std::memcpy(out->pospres + 4 * (size_t)t, &pp[k], 16);
         if (out->velrhop) std::memcpy(out->velrhop + 4 * (size_t)t, &vr[k], 16);
         if (out->idep) std::memcpy(out->idep + 4 * (size_t)t, &id[k], 16);
@@ -1046,5 +1091,51 @@ extern "C" lq_status lq_engine_download(lq_engine* e, int32_t run_id, lq_host_pa
             posd[3 * (size_t)t + 2] = zd[k];
         }
     }
+    // particles removed by lq_engine_compact: NaN positions, other fields untouched
+    const float fnan = std::numeric_limits<float>::quiet_NaN();
+    for (int t = 0; t < n_in; ++t) {
+        if (seen[t]) continue;
+        if (out->pospres)
+            for (int a = 0; a < 3; ++a) out->pospres[4 * (size_t)t + a] = fnan;
+        if (posd)
+            for (int a = 0; a < 3; ++a) posd[3 * (size_t)t + a] = std::numeric_limits<double>::quiet_NaN();
+    }
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_engine_compact(lq_engine* e, int64_t* removed) {
+    if (!e) {
+        set_error("null engine");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (removed) *removed = 0;
+    if (e->runs.empty()) return LQ_OK;
+    LQ_TRY(sync_descs(e));
+    // after a sort, a run's particles outside its grid (cell ncell - 1) are
+    // the tail of its segment
+    sort_resident(e);
+    LQ_HIP_TRY(hipStreamSynchronize(e->stream));
+    int64_t total = 0;
+    for (RunSlot& r : e->runs) {
+        RunDesc& d = r.desc;
+        const int64_t cell = (int64_t)d.cell_base + d.par.ncell - 1;
+        int32_t be[2] = {-1, -1};
+        LQ_HIP_TRY(hipMemcpy(&be[0], e->b.grid.cell_beg + cell, 4, hipMemcpyDeviceToHost));
+        LQ_HIP_TRY(hipMemcpy(&be[1], e->b.grid.cell_end + cell, 4, hipMemcpyDeviceToHost));
+        if (be[0] < 0) continue;
+        const int32_t ndel = be[1] - be[0];
+        if (be[1] != d.p_off + d.np || be[0] < d.p_off + d.npb) {
+            set_error("lq_engine_compact: particles outside the grid are not the material tail of their run");
+            return LQ_ERR_STATE;
+        }
+        d.np -= ndel;
+        d.npm -= ndel;
+        d.par.np -= ndel;
+        d.par.npm -= ndel;
+        d.par.np_delete += ndel;
+        total += ndel;
+    }
+    if (removed) *removed = total;
+    if (total > 0) LQ_TRY(relayout(e));
     return LQ_OK;
 }

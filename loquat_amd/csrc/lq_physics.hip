@@ -647,6 +647,9 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
         const uint32_t jlast = (uint32_t)max(je - 1, 0);
         while (true) {
             while (__ballot(j < je) && !__ballot(cnt > IC2_CAP - U)) {
+                // a lane whose column is done issues no loads (module calls
+                // keep every lane in step: their loads are clamped instead)
+                if (PADDED && j >= je) continue;
                 float4 q[U];
                 if (PADDED) {
                     // one offset for the four loads; a finished lane re-reads
@@ -1217,6 +1220,73 @@ __global__ void k_step_end(RunDesc* d, int nruns, const int32_t* __restrict__ fa
     d[r].time += (double)d[r].par.dt;
     d[r].step += 1;
     if (d[r].step >= d[r].nsteps) d[r].active = 0;
+    // the reference's loop condition, src/loquat.cu:322
+    if (d[r].until_time && d[r].time >= (double)d[r].par.time_max) d[r].active = 0;
+}
+
+// Dynamic time step, part 1: per-run maxima of |v| over the material
+// particles of the current state and of |a| over the rates of the previous
+// step's corrector (ignored at a run's first step, when there are none).
+// Non-negative floats order as their bit patterns, so the maxima are
+// unsigned atomics: one per wave when the wave lies in one run.
+__global__ __launch_bounds__(256) void k_dt_reduce(const RunDesc* __restrict__ d, const int32_t* __restrict__ run_of,
+                                                   const float4* __restrict__ velrhop,
+                                                   const float4* __restrict__ accad, IndexMap mats, int n,
+                                                   uint32_t* __restrict__ red) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = -1;
+    float vm = 0.0f, am = 0.0f;
+    if (t < n) {
+        const int k = mats(t);
+        r = run_of[k];
+        const RunDesc& dr = d[r];
+        if (dr.active && dr.par.dynamic_time_step == 1) {
+            const float4 v = velrhop[k];
+            vm = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+            if (dr.step > 0) {
+                const float4 acc = accad[k];
+                am = sqrtf(acc.x * acc.x + acc.y * acc.y + acc.z * acc.z);
+            }
+        } else {
+            r = -1;
+        }
+    }
+    // wave-wide maxima when every contributing lane has the same run
+    const unsigned long long has = __ballot(r >= 0);
+    if (has == 0ull) return;  // uniform over the wave
+    const int r0 = __shfl(r, __ffsll((long long)has) - 1, 64);
+    const bool same = __ballot(r >= 0 && r != r0) == 0ull;
+    if (same) {
+        for (int off = 32; off > 0; off >>= 1) {
+            vm = fmaxf(vm, __shfl_xor(vm, off, 64));
+            am = fmaxf(am, __shfl_xor(am, off, 64));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMax(&red[2 * r0], __float_as_uint(vm));
+            atomicMax(&red[2 * r0 + 1], __float_as_uint(am));
+        }
+    } else if (r >= 0) {
+        atomicMax(&red[2 * r], __float_as_uint(vm));
+        atomicMax(&red[2 * r + 1], __float_as_uint(am));
+    }
+}
+
+// Dynamic time step, part 2: dt = cfl * min(h / (cs + |v|max), sqrt(h / |a|max))
+// for each run with dynamic-time-step = 1 (no term for |a|max = 0), then
+// clears the maxima for the next step.
+__global__ void k_dt_update(RunDesc* d, int nruns, uint32_t* __restrict__ red) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nruns) return;
+    RunDesc& dr = d[r];
+    if (dr.active && dr.par.dynamic_time_step == 1) {
+        const float vmax = __uint_as_float(red[2 * r]), amax = __uint_as_float(red[2 * r + 1]);
+        const float h = dr.par.h;
+        float dt = h / (dr.par.cs + vmax);
+        if (amax > 0.0f) dt = fminf(dt, sqrtf(h / amax));
+        dr.par.dt = dr.par.cfl * dt;
+    }
+    red[2 * r] = 0u;
+    red[2 * r + 1] = 0u;
 }
 
 __global__ void k_copy_tensor(const tensor2d* __restrict__ src, tensor2d* __restrict__ dst, int n) {
@@ -1319,6 +1389,11 @@ void launch_shepard_table(const RunDesc* d, const int32_t* run_of, const DevStat
 
 void launch_step_begin(RunDesc* d, int nruns, hipStream_t s) {
     if (nruns > 0) k_step_begin<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns);
+}
+void launch_dynamic_dt(RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt, IndexMap mats,
+                       int nmat, int nruns, uint32_t* red, hipStream_t s) {
+    if (nmat > 0) k_dt_reduce<<<blocks_for(nmat, 256), 256, 0, s>>>(d, run_of, st.velrhop, rt.accad, mats, nmat, red);
+    if (nruns > 0) k_dt_update<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns, red);
 }
 void launch_step_end(RunDesc* d, int nruns, const int32_t* fail, hipStream_t s) {
     if (nruns > 0) k_step_end<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns, fail);

@@ -219,6 +219,12 @@ class Engine:
     def set_graphs(self, on: bool) -> None:
         abi.check(abi.lib().lq_engine_set_graphs(self.h, 1 if on else 0))
 
+    def compact(self) -> int:
+        """Removes material particles that left their run's grid; returns how many."""
+        n = C.c_int64()
+        abi.check(abi.lib().lq_engine_compact(self.h, C.byref(n)))
+        return n.value
+
     def set_cell_order(self, zbin_bits: int) -> None:
         """0: the reference's in-cell order; b > 0: cells sorted by 2^b z-bins."""
         abi.check(abi.lib().lq_engine_set_cell_order(self.h, int(zbin_bits)))

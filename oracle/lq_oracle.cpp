@@ -591,7 +591,30 @@ void lqo_initial_sort(const lq_params* par, lqo_state* st) {
 
 // nsteps steps of the time loop (src/loquat.cu:322-409) starting at step
 // index step0 (the Shepard cadence counts from there).
-void lqo_run(const lq_params* par, const lq_vw_params* vw, lqo_state* st, int nsteps, int step0) {
+// Dynamic time step (dynamic-time-step = 1).  Specification decision: the
+// reference's call is commented out (src/loquat.cu:325-327) and its body is
+// not shipped.  dt = cfl * min(h / (cs + max|v|), sqrt(h / max|a|)) over the
+// material particles, with a = the previous corrector's rates; the |a| term
+// is absent when use_rates is 0 (a run's first step) or max|a| = 0.  The
+// result is rounded to FP32, the type of Parameters::dt.
+double lqo_dynamic_dt(const lq_params* par, const lqo_state* st, const lqo_rates* rt, int use_rates) {
+    double vmax = 0.0, amax = 0.0;
+    for (int i = par->npb; i < par->np; ++i) {
+        vmax = std::max(vmax, std::sqrt(st->vx[i] * st->vx[i] + st->vy[i] * st->vy[i] + st->vz[i] * st->vz[i]));
+        if (use_rates)
+            amax = std::max(amax, std::sqrt(rt->ax[i] * rt->ax[i] + rt->ay[i] * rt->ay[i] + rt->az[i] * rt->az[i]));
+    }
+    const double h = par->h;
+    double dt = h / ((double)par->cs + vmax);
+    if (amax > 0.0) dt = std::min(dt, std::sqrt(h / amax));
+    return (double)(float)((double)par->cfl * dt);
+}
+
+// lqo_run, also writing each step's dt to dts (nullable).
+void lqo_run_dt(const lq_params* par_in, const lq_vw_params* vw, lqo_state* st, int nsteps, int step0,
+                double* dts) {
+    lq_params pl = *par_in;  // dt changes per step with dynamic-time-step = 1
+    const lq_params* par = &pl;
     const int n = par->np;
     std::vector<int32_t> perm(n), sorted(n), beg(par->ncell), end(par->ncell);
     std::vector<double> ib((size_t)21 * n), rb((size_t)13 * n);
@@ -610,6 +633,8 @@ void lqo_run(const lq_params* par, const lq_vw_params* vw, lqo_state* st, int ns
     for (int c = 0; c < 6; ++c) rt.d[c] = rb.data() + (size_t)(k++) * n;
     for (int it = 0; it < nsteps; ++it) {
         const int step = step0 + it;
+        if (pl.dynamic_time_step == 1) pl.dt = (float)lqo_dynamic_dt(&pl, st, &rt, it > 0 ? 1 : 0);
+        if (dts) dts[it] = (double)pl.dt;
         build_and_sort(par, st, nullptr, perm, sorted, beg, end);
         lqo_adami(par, st, beg.data(), end.data());
         lqo_interaction(par, st, &rt, beg.data(), end.data(), nullptr);
@@ -624,6 +649,10 @@ void lqo_run(const lq_params* par, const lq_vw_params* vw, lqo_state* st, int ns
             lqo_shepard(par, st, beg.data(), end.data());
         }
     }
+}
+
+void lqo_run(const lq_params* par, const lq_vw_params* vw, lqo_state* st, int nsteps, int step0) {
+    lqo_run_dt(par, vw, st, nsteps, step0, nullptr);
 }
 
 // One constitutive update at a single material point (the constitutive

@@ -54,6 +54,9 @@ def lib() -> C.CDLL:
         L.lqo_shepard.argtypes = [vp, C.POINTER(_State), ip, ip]
         L.lqo_initial_sort.argtypes = [vp, C.POINTER(_State)]
         L.lqo_run.argtypes = [vp, vp, C.POINTER(_State), C.c_int, C.c_int]
+        L.lqo_run_dt.argtypes = [vp, vp, C.POINTER(_State), C.c_int, C.c_int, dp]
+        L.lqo_dynamic_dt.argtypes = [vp, C.POINTER(_State), C.POINTER(_Rates), C.c_int]
+        L.lqo_dynamic_dt.restype = C.c_double
         L.lqo_num_threads.restype = C.c_int
         L.lqo_set_num_threads.argtypes = [C.c_int]
         L.lqo_constitutive.argtypes = [vp, vp, C.c_double, dp, dp, dp, dp, dp, dp, dp]
@@ -256,6 +259,19 @@ def initial_sort(par, st: OState) -> None:
 def run(par, vw, st: OState, nsteps: int, step0: int = 0) -> None:
     s = st.struct()
     lib().lqo_run(_p(par), _p(vw), C.byref(s), int(nsteps), int(step0))
+
+
+def run_dt(par, vw, st: OState, nsteps: int, step0: int = 0) -> np.ndarray:
+    """run(), returning the dt of every step (varies with dynamic-time-step = 1)."""
+    dts = np.zeros(max(nsteps, 1), np.float64)
+    s = st.struct()
+    lib().lqo_run_dt(_p(par), _p(vw), C.byref(s), int(nsteps), int(step0), _d(dts))
+    return dts[:nsteps]
+
+
+def dynamic_dt(par, st: OState, rt: ORates, use_rates: bool) -> float:
+    s, r = st.struct(), rt.struct()
+    return lib().lqo_dynamic_dt(_p(par), C.byref(s), C.byref(r), 1 if use_rates else 0)
 
 
 def num_threads() -> int:

@@ -264,3 +264,88 @@ def test_restart_from_output_continues_the_run(gpu, tmp_path):
     assert rel_err(got["posd"] - x0, ref["posd"] - x0) < 1e-5
     for k in ("velrhop", "str"):
         assert rel_err(got[k], ref[k]) < 1e-5, k
+
+
+@pytest.mark.gpu
+def test_dynamic_time_step_matches_oracle(gpu):
+    """dynamic-time-step = 1: the engine's per-step dt (device max-reductions
+    of |v| and |a|) and its trajectory follow the oracle's rule."""
+    case = gentle_case(1, seed=91)
+    case.par.dynamic_time_step = 1
+    case.par.stress_shepard_filter = 5
+    nsteps = 20
+    e, (rid,) = _engine([case], nsteps)
+    e.step(nsteps)
+    got = e.download(rid)
+    info = e.info(rid)
+    e.close()
+    p = case.derived()
+    st = O.OState.from_case(case)
+    O.initial_sort(p, st)
+    dts = O.run_dt(p, case.vw, st, nsteps)
+    ref = O.unpermute(st)
+    assert len(set(dts.tolist())) > 1                        # the step did change
+    assert info["steps"] == nsteps
+    assert info["time"] - p.time == pytest.approx(dts.sum(), rel=1e-6)
+    m = slice(p.npb, None)
+    x0 = case.posd
+    for a, k in enumerate(("x", "y", "z")):
+        assert rel_err(got["posd"][m, a] - x0[m, a], getattr(ref, k)[m] - x0[m, a]) < 1e-4, k
+    for a, k in enumerate(("vx", "vy", "vz")):
+        assert rel_err(got["velrhop"][m, a], getattr(ref, k)[m]) < 1e-4, k
+
+
+@pytest.mark.gpu
+def test_dynamic_run_ends_at_time_max(gpu):
+    """A run added with nsteps = 0 and a dynamic step ends on the device once
+    time >= time-max (src/loquat.cu:322), batched with a fixed-step run that
+    keeps going."""
+    case = gentle_case(1, seed=93)
+    case.par.dynamic_time_step = 1
+    p = case.derived()
+    case.par.time_max = p.time + 12.5 * p.dt
+    other = gentle_case(1, seed=94)
+    e, _ = _engine([], spare=(case, other))
+    rid = e.add(case, 0)        # 0: run until time-max
+    oid = e.add(other, 40)
+    e.step(40)
+    info, oinfo = e.info(rid), e.info(oid)
+    e.close()
+    assert oinfo["steps"] == 40 and oinfo["finished"]
+    st = O.OState.from_case(case)
+    O.initial_sort(p, st)
+    dts = O.run_dt(p, case.vw, st, 40)
+    t = p.time + np.cumsum(dts)
+    expect = int(np.argmax(t >= case.par.time_max)) + 1
+    assert info["finished"] and info["steps"] == expect
+    assert info["time"] >= case.par.time_max
+
+
+@pytest.mark.gpu
+def test_compact_removes_particles_outside_the_grid(gpu):
+    """Material particles outside the grid are removed by lq_engine_compact;
+    every other particle ends bit-identical to the same run without
+    compaction (no neighbour search ever finds the removed ones)."""
+    case = cases.perturb(cases.small_column_3d(seed=101), seed=102)
+    p = case.derived()
+    gone = p.npb + np.arange(5, 5 + 7 * 37, 37)           # seven material particles
+    case.posd[gone, 2] = p.domain_size_z + 0.5              # above the grid
+    case.pospres[gone, 2] = case.posd[gone, 2].astype(np.float32)
+    e, (rid,) = _engine([case], 12)
+    e.step(12)
+    ref = e.download(rid)
+    e.close()
+    e, (rid,) = _engine([case], 12)
+    e.step(4)
+    assert e.compact() == len(gone)
+    assert e.totals()["np"] == case.np - len(gone)
+    e.step(8)
+    got = e.download(rid)
+    info = e.info(rid)
+    e.close()
+    assert info["steps"] == 12 and info["np"] == case.np
+    keep = np.ones(case.np, bool)
+    keep[gone] = False
+    assert np.isnan(got["pospres"][gone, :3]).all() and np.isnan(got["posd"][gone]).all()
+    for k in ("posd", "pospres", "velrhop", "idep", "str", "stn"):
+        assert np.array_equal(got[k][keep], ref[k][keep]), k

@@ -384,3 +384,45 @@ def test_midpoint_integration_exact_for_constant_acceleration():
     dt = float(p.dt)
     assert np.allclose(st.x, x0 + vx0 * dt + 0.5 * 2.0 * dt * dt, rtol=1e-14)
     assert np.allclose(st.vx, vx0 + 2.0 * dt, rtol=1e-14)
+
+
+def test_dynamic_time_step_rule():
+    """dt = cfl * min(h / (cs + max|v|), sqrt(h / max|a|)) over material
+    particles: at rest it is the constant step cfl*h/cs; it shrinks with the
+    fastest material particle and not with walls; a large acceleration takes
+    over; the |a| term is absent at a run's first step."""
+    case = cases.small_column_3d(seed=3)
+    p = case.derived()
+    st = O.OState.from_case(case)
+    rt = O.ORates(st.n)
+    npb = p.npb
+    dt0 = O.dynamic_dt(p, st, rt, False)
+    assert dt0 == pytest.approx(p.cfl * p.h / p.cs, rel=1e-7)
+    st.vx[:npb] = 100.0                       # walls do not count
+    assert O.dynamic_dt(p, st, rt, False) == dt0
+    st.vx[npb + 7] = -3.0
+    st.vz[npb + 7] = 4.0                      # |v| = 5
+    assert O.dynamic_dt(p, st, rt, False) == pytest.approx(p.cfl * p.h / (p.cs + 5.0), rel=1e-7)
+    rt.az[npb + 11] = -9.81
+    dt_v = O.dynamic_dt(p, st, rt, True)
+    assert dt_v == pytest.approx(p.cfl * p.h / (p.cs + 5.0), rel=1e-7)   # gravity: sqrt(h/g) >> h/cs
+    big = 4.0 * p.h / (p.h / (p.cs + 5.0)) ** 2                        # sqrt(h/big) = half the velocity bound
+    rt.ax[npb + 20] = big
+    assert O.dynamic_dt(p, st, rt, True) == pytest.approx(p.cfl * math.sqrt(p.h / big), rel=1e-6)
+    assert O.dynamic_dt(p, st, rt, False) == pytest.approx(p.cfl * p.h / (p.cs + 5.0), rel=1e-7)
+
+
+def test_dynamic_run_steps_shrink_as_the_column_moves():
+    """A collapsing column run with dynamic-time-step = 1: every dt is at most
+    the constant step, the first equals it (at rest), later ones fall as the
+    material speeds up."""
+    case = cases.small_column_3d(nx=8, ny=4, nz=12, seed=4)
+    case.par.dynamic_time_step = 1
+    p = case.derived()
+    st = O.OState.from_case(case)
+    O.initial_sort(p, st)
+    dts = O.run_dt(p, case.vw, st, 30)
+    dt_const = np.float32(p.cfl * p.h / p.cs)
+    assert dts[0] == pytest.approx(dt_const, rel=1e-7)
+    assert (dts <= dt_const * (1 + 1e-7)).all()
+    assert dts[-1] < dts[0]
