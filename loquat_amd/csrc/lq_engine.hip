@@ -27,7 +27,6 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 
 static int g_variant = LQ_DEFAULT_INTERACTION_VARIANT;
-int interaction_variant() { return g_variant; }
 
 // Smallest FP32 value T with sqrtf(T) * i_h >= 2 (both correctly rounded):
 // r2 < T is then exactly the predicate q = sqrtf(r2) * i_h < 2 for every

@@ -115,6 +115,5 @@ void launch_dynamic_dt(RunDesc* d, const int32_t* run_of, const DevState& st, co
 // Copies material stress back (reference's D2D copy, src/field_smoothing.cu:144).
 void launch_copy_tensor(const tensor2d* src, tensor2d* dst, int n, hipStream_t s);
 
-int interaction_variant();
 
 }  // namespace lq

@@ -133,9 +133,11 @@ struct IConst {
     float i_h, cg, h, eta, alpha_cs;
 };
 
-// One pair i-j (Appendix formulation: Wendland C2 gradient, SPH velocity
-// gradient, continuity, momentum with sigma_i/rho_i^2 + sigma_j/rho_j^2 and
-// Monaghan viscosity acting as a pressure).
+// One pair i-j: Wendland C2 gradient dW = C_g (2 - q)^3 (x_i - x_j), SPH
+// velocity gradient L_ab += (v_j - v_i)_a / rho_j dW_b, continuity
+// drho += (v_i - v_j).dW, momentum a += (sigma_i/rho_i^2 + sigma_j/rho_j^2
+// - Pi_ij I).dW with Monaghan's viscosity Pi_ij for approaching pairs (the
+// mass m multiplies the sums when they are stored).
 __device__ __forceinline__ void pair_eval(const IPart& pi, int j, const DevState& st, const IConst& c, IAcc& a) {
     const float dx = (float)(pi.x - st.xd[j]), dy = (float)(pi.y - st.yd[j]), dz = (float)(pi.z - st.zd[j]);
     const float r2 = dx * dx + dy * dy + dz * dz;
