@@ -349,3 +349,48 @@ def test_compact_removes_particles_outside_the_grid(gpu):
     assert np.isnan(got["pospres"][gone, :3]).all() and np.isnan(got["posd"][gone]).all()
     for k in ("posd", "pospres", "velrhop", "idep", "str", "stn"):
         assert np.array_equal(got[k][keep], ref[k][keep]), k
+
+
+@pytest.mark.gpu
+def test_engine_edge_cases_and_errors(gpu):
+    """A walls-only run beside a normal one, a run of zero steps, capacity and
+    id errors, and stepping an engine whose runs have all finished."""
+    from loquat_amd.device import Engine
+    normal = cases.perturb(cases.small_column_3d(seed=111), seed=112)
+    walls = cases.small_column_3d(seed=113)
+    npb = walls.derived().npb
+    keep = slice(0, npb)
+    walls = cases.Case("walls-only", walls.par.copy(), walls.vw, walls.pospres[keep].copy(),
+                       walls.velrhop[keep].copy(), walls.idep[keep].copy(), walls.str[keep].copy(),
+                       walls.stn[keep].copy(), walls.posd[keep].copy(), 5)
+    walls.par.np, walls.par.npm = npb, 0
+    alone, (ra,) = _engine([normal], 5)
+    alone.step(5)
+    ref = alone.download(ra)
+    alone.close()
+    e = Engine(normal.np + walls.np, normal.derived().ncell + walls.derived().ncell, max_runs=2)
+    rw = e.add(walls, 5)
+    rn = e.add(normal, 5)
+    with pytest.raises(abi.LqError) as err:       # run table full
+        e.add(normal, 5)
+    assert err.value.status == 8
+    e.step(9)                                     # runs stop at their own 5 steps
+    assert e.info(rn)["steps"] == 5 and e.info(rn)["finished"]
+    assert e.info(rw)["steps"] == 5
+    assert _same(e.download(rn), ref)
+    got_w = e.download(rw)
+    assert np.array_equal(got_w["posd"], walls.posd)      # walls never move
+    e.step(3)                                     # nothing left to run
+    assert e.info(rn)["steps"] == 5
+    with pytest.raises(abi.LqError) as err:
+        e.info(12345)
+    assert err.value.status == 1
+    e.remove(rw)
+    assert e.totals()["nruns"] == 1
+    assert _same(e.download(rn), ref)
+    e.close()
+    small = Engine(10, 10, 1)
+    with pytest.raises(abi.LqError) as err:       # more particles than the engine holds
+        small.add(normal, 1)
+    assert err.value.status == 8
+    small.close()
