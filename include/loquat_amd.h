@@ -212,7 +212,11 @@ lq_status lq_adami_boundary(const lq_params* par, const lq_state* st,
 
 /* ParticleInteraction (src/particle_interaction.h:29-41): accad, spin and
  * acestn of material particles [npb, np).  nbr_count (nullable, int32[np])
- * receives each material particle's neighbour count (j != i, q < 2). */
+ * receives each material particle's neighbour count (j != i, q < 2).
+ * artificial-pressure = 1 adds Monaghan's tensile-instability term (J. Comput.
+ * Phys. 159, 2000, exponent 4) to the pair's viscous pressure:
+ * (R_i + R_j) (W(r_ij) / W(dr))^4 with R = coefficient * max(-p, 0) / rho^2,
+ * p = pospres.w, W(dr) = par->kernel_dr (src/loquat.cu:133-138). */
 lq_status lq_particle_interaction(const lq_params* par, const lq_state* st, const lq_rates* rates,
                                   const int32_t* cell_beg, const int32_t* cell_end,
                                   int32_t* nbr_count, lq_stream stream);

@@ -98,9 +98,9 @@ static lq_status check_params(const lq_params* par) {
         set_error("h / m not derived (call lq_params_derive)");
         return LQ_ERR_INVALID_ARG;
     }
-    if (par->artificial_pressure == 1) {
-        set_error("artificial-pressure = 1 is not implemented");
-        return LQ_ERR_UNSUPPORTED;
+    if (par->artificial_pressure == 1 && !(par->kernel_dr > 0.0f)) {
+        set_error("kernel-dr not derived for artificial-pressure = 1 (call lq_params_derive)");
+        return LQ_ERR_INVALID_ARG;
     }
     if (par->constitutive_model < 1 || par->constitutive_model > 3) {
         set_error("constitutive-model must be 1 (Drucker-Prager), 2 (Wu hypoplastic) or 3 (von Wolffersdorff)");
@@ -356,6 +356,7 @@ struct lq_engine {
     int variant = -1;  // interaction variant pinned by lq_engine_autotune (-1: library default)
     int zbits = 4;     // z-bins per cell (log2) in the engine's sort key
     bool any_dynamic = false;  // some resident run has dynamic-time-step = 1
+    int64_t nmat_art = 0;      // material particles of runs with artificial-pressure = 1 (last in mat_idx)
     hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 };
 
@@ -434,7 +435,7 @@ void enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
         {
             StageTimer t(prof, s, PG_INTER);
             launch_interaction_table(b.desc, b.run_of, b.S[c], b.R, b.grid.cell_beg, b.grid.cell_end, mats, nm,
-                                     nullptr, var, s);
+                                     (int)e->nmat_art, nullptr, var, s);
         }
         {
             StageTimer t(prof, s, PG_INTEG);
@@ -501,7 +502,7 @@ lq_status sync_descs(lq_engine* e) {
 lq_status upload_descs(lq_engine* e) {
     std::vector<RunDesc> h(e->runs.size());
     std::vector<int32_t> f(e->runs.size());
-    std::vector<int32_t> walls, mats;
+    std::vector<int32_t> walls, mats, mats_art;
     const int zb = zbits_eff(e);
     e->any_dynamic = false;
     for (size_t i = 0; i < e->runs.size(); ++i) {
@@ -511,8 +512,11 @@ lq_status upload_descs(lq_engine* e) {
         h[i] = d;
         f[i] = e->runs[i].failed;
         for (int k = 0; k < d.npb; ++k) walls.push_back(d.p_off + k);
-        for (int k = d.npb; k < d.np; ++k) mats.push_back(d.p_off + k);
+        std::vector<int32_t>& m = d.par.artificial_pressure == 1 ? mats_art : mats;
+        for (int k = d.npb; k < d.np; ++k) m.push_back(d.p_off + k);
     }
+    e->nmat_art = (int64_t)mats_art.size();
+    mats.insert(mats.end(), mats_art.begin(), mats_art.end());
     e->nwall_total = (int64_t)walls.size();
     e->nmat_total = (int64_t)mats.size();
     if (!h.empty()) {
@@ -865,7 +869,7 @@ extern "C" lq_status lq_engine_neighbour_counts(lq_engine* e, int32_t run_id, in
     const IndexMap mats{b.mat_idx, 0};
     const int var = e->variant >= 0 ? e->variant : g_variant;
     launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
-                             (int)e->nmat_total, nbr, var, s);
+                             (int)e->nmat_total, (int)e->nmat_art, nbr, var, s);
     const RunDesc& d = e->runs[i].desc;
     std::vector<int32_t> c(d.np), orig(d.np);
     hipError_t err = hipGetLastError();
@@ -910,11 +914,11 @@ extern "C" lq_status lq_engine_autotune(lq_engine* e, int32_t nrep, int32_t* bes
     int best_v = 3;
     for (int v = 0; v < LQ_INTERACTION_VARIANTS; ++v) {
         launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
-                                 (int)e->nmat_total, nullptr, v, s);  // warm-up
+                                 (int)e->nmat_total, (int)e->nmat_art, nullptr, v, s);  // warm-up
         LQ_HIP_TRY(hipEventRecord(ev[0], s));
         for (int r = 0; r < nrep; ++r)
             launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
-                                     (int)e->nmat_total, nullptr, v, s);
+                                     (int)e->nmat_total, (int)e->nmat_art, nullptr, v, s);
         LQ_HIP_TRY(hipEventRecord(ev[1], s));
         LQ_HIP_TRY(hipEventSynchronize(ev[1]));
         float ms = 0.f;

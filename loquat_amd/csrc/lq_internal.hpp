@@ -86,9 +86,12 @@ void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState&
                         const int32_t* end, IndexMap walls, int nwalls, int variant, hipStream_t s);
 void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates& rt, const int32_t* beg,
                             const int32_t* end, int32_t* nbr, int variant, hipStream_t s);
+// The last nart of the nmat material indices belong to runs with
+// artificial-pressure = 1; they get a kernel of their own so that the other
+// runs keep the exact arithmetic of the plain kernel.
 void launch_interaction_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt,
-                              const int32_t* beg, const int32_t* end, IndexMap mats, int nmat, int32_t* nbr,
-                              int variant, hipStream_t s);
+                              const int32_t* beg, const int32_t* end, IndexMap mats, int nmat, int nart,
+                              int32_t* nbr, int variant, hipStream_t s);
 void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, const DevIni& ini,
                           const DevRates& rt, hipStream_t s);
 // ini_perm: the corrector reads particle k's ini entry at ini_perm[k] (the

@@ -124,6 +124,7 @@ struct IPart {
     double x, y, z;
     float vx, vy, vz, rho;
     tensor2d T;  // sigma_i / rho_i^2
+    float R;     // artificial pressure eps * max(-p_i, 0) / rho_i^2 (0 when off)
 };
 struct IAcc {
     float ax, ay, az, dvg;
@@ -131,6 +132,7 @@ struct IAcc {
 };
 struct IConst {
     float i_h, cg, h, eta, alpha_cs;
+    float eps, cw, inv_wdr;  // artificial pressure: coefficient (0 when off), C_w, 1 / W(dr)
 };
 
 // One pair i-j: Wendland C2 gradient dW = C_g (2 - q)^3 (x_i - x_j), SPH
@@ -198,6 +200,10 @@ __device__ __forceinline__ IConst make_iconst(const RunDesc& d) {
     c.h = d.par.h;
     c.eta = d.par.eta;
     c.alpha_cs = d.par.viscosity_type == 1 ? d.par.artificial_viscosity_alpha * d.par.cs : 0.0f;
+    const bool art = d.par.artificial_pressure == 1 && d.par.kernel_dr > 0.0f;
+    c.eps = art ? d.par.artificial_pressure_coefficient : 0.0f;
+    c.cw = d.par.kernel_normalization_par;
+    c.inv_wdr = art ? 1.0f / d.par.kernel_dr : 0.0f;
     return c;
 }
 
@@ -477,14 +483,20 @@ struct PairIn {
     double x, y, z;
     float4 vr;
     tensor2d s;
+    float p;  // pressure of j, read only with artificial pressure
 };
 
+template <bool ART = false>
 __device__ __forceinline__ PairIn pair_load(const DevState& st, uint32_t j) {
-    return PairIn{ldo(st.xd, j), ldo(st.yd, j), ldo(st.zd, j), ldo(st.velrhop, j), ldo(st.str, j)};
+    const float pj = ART ? ldo(reinterpret_cast<const float*>(st.pospres) + 3, 4u * j) : 0.0f;
+    return PairIn{ldo(st.xd, j), ldo(st.yd, j), ldo(st.zd, j), ldo(st.velrhop, j), ldo(st.str, j), pj};
 }
 
-// Pair body of variants 3 and 4: as pair_eval, with 32-bit offset loads and
-// the viscosity term evaluated branch-free.
+// Pair body of variants 3 to 6: as pair_eval, with 32-bit offset loads and
+// the viscosity term evaluated branch-free.  ART adds Monaghan's artificial
+// pressure (R_i + R_j) f^4, f = W(r) / W(dr), R = eps max(-p, 0) / rho^2, to
+// the viscous pressure (enabled by artificial-pressure = 1).
+template <bool ART = false>
 __device__ __forceinline__ void pair_compute(const IPart& pi, const PairIn& pj, const IConst& c, IAcc& a) {
     const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y), dz = (float)(pi.z - pj.z);
     const float r2 = dx * dx + dy * dy + dz * dz;
@@ -509,8 +521,14 @@ __device__ __forceinline__ void pair_compute(const IPart& pi, const PairIn& pj, 
     a.dvg += dvx * gx + dvy * gy + dvz * gz;
     const float vdx = -(dvx * dx + dvy * dy + dvz * dz);
     const float mu = c.h * fminf(vdx, 0.0f) * frcp(r2 + c.eta);
-    const float pij = -2.0f * c.alpha_cs * mu * frcp(pi.rho + vr.w);
+    float pij = -2.0f * c.alpha_cs * mu * frcp(pi.rho + vr.w);
     const float ir2 = ir * ir;
+    if constexpr (ART) {
+        const float t2 = t * t;
+        const float f = c.cw * t2 * t2 * (q + 0.5f) * c.inv_wdr;
+        const float f2 = f * f;
+        pij += (pi.R + c.eps * fmaxf(-pj.p, 0.0f) * ir2) * (f2 * f2);
+    }
     const float sxx = pi.T.xx + sj.xx * ir2 - pij;
     const float syy = pi.T.yy + sj.yy * ir2 - pij;
     const float szz = pi.T.zz + sj.zz * ir2 - pij;
@@ -697,7 +715,7 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
 // zeroes the kernel gradient and v_i - v_i the velocity terms), so it changes
 // no sum, and one is taken off the neighbour count.  Variant 4 (B = 2)
 // evaluates the lists two pairs at a time; same pairs, same order.
-template <class DP, int B, bool MASKED = false, int MINW = 1, int U = IC2_U>
+template <class DP, int B, bool MASKED = false, int MINW = 1, int U = IC2_U, bool ART = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(MINW))) void k_interaction_v3(
     DP dp, DevState st, DevRates rt,
                                                         const int32_t* __restrict__ beg,
@@ -735,28 +753,29 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_
     // MASKED: a lane past the end of its list loads nothing and computes the
     // self pair from registers (positions and velocity of i, zero stress),
     // instead of gathering particle i's data again
-    const PairIn self = PairIn{pi.x, pi.y, pi.z, make_float4(pi.vx, pi.vy, pi.vz, pi.rho), tensor2d{}};
+    const PairIn self = PairIn{pi.x, pi.y, pi.z, make_float4(pi.vx, pi.vy, pi.vz, pi.rho), tensor2d{}, ps.w};
+    if constexpr (ART) pi.R = c.eps * fmaxf(-ps.w, 0.0f) / (pi.rho * pi.rho);
     auto load = [&](uint32_t j, bool act) {
         if constexpr (MASKED) {
             PairIn q = self;
-            if (act) q = pair_load(st, j);
+            if (act) q = pair_load<ART>(st, j);
             return q;
         } else {
-            return pair_load(st, j);
+            return pair_load<ART>(st, j);
         }
     };
     if constexpr (B == 1) {
         count = scan_compact<1, DP::padded, U>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
                                             mat ? (uint32_t)k : 0u,
-                                            [&](uint32_t j, bool act) { pair_compute(pi, load(j, act), c, a); });
+                                            [&](uint32_t j, bool act) { pair_compute<ART>(pi, load(j, act), c, a); });
     } else {
         count = scan_compact<2, DP::padded, U>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
                                             mat ? (uint32_t)k : 0u,
                                             [&](uint32_t j0, bool a0, uint32_t j1, bool a1) {
                                                 const PairIn q0 = load(j0, a0);
                                                 const PairIn q1 = load(j1, a1);
-                                                pair_compute(pi, q0, c, a);
-                                                pair_compute(pi, q1, c, a);
+                                                pair_compute<ART>(pi, q0, c, a);
+                                                pair_compute<ART>(pi, q1, c, a);
                                             });
     }
     if (mat) {
@@ -1320,10 +1339,13 @@ void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState&
 
 template <class DP>
 static void launch_interaction(const DP& dp, const DevState& st, const DevRates& rt, const int32_t* beg,
-                               const int32_t* end, IndexMap mats, int n, int32_t* nbr, int variant, hipStream_t s) {
+                               const int32_t* end, IndexMap mats, int n, int32_t* nbr, int variant, bool art,
+                               hipStream_t s) {
     if (n <= 0) return;
     const int nb = blocks_for(n, 256);
-    if (variant == 0)
+    if (art)  // artificial pressure: one kernel (variant 6's) carries the extra term
+        k_interaction_v3<DP, 2, true, 1, IC2_U, true><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+    else if (variant == 0)
         k_interaction_direct<DP><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else if (variant == 1)
         k_interaction_lds<DP><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
@@ -1341,12 +1363,16 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
 
 void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates& rt, const int32_t* beg,
                             const int32_t* end, int32_t* nbr, int variant, hipStream_t s) {
-    launch_interaction(DescOne{d}, st, rt, beg, end, IndexMap{nullptr, d.npb}, d.np - d.npb, nbr, variant, s);
+    launch_interaction(DescOne{d}, st, rt, beg, end, IndexMap{nullptr, d.npb}, d.np - d.npb, nbr, variant,
+                       d.par.artificial_pressure == 1, s);
 }
 void launch_interaction_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt,
-                              const int32_t* beg, const int32_t* end, IndexMap mats, int nmat, int32_t* nbr,
-                              int variant, hipStream_t s) {
-    launch_interaction(DescTable{d, run_of}, st, rt, beg, end, mats, nmat, nbr, variant, s);
+                              const int32_t* beg, const int32_t* end, IndexMap mats, int nmat, int nart,
+                              int32_t* nbr, int variant, hipStream_t s) {
+    const DescTable dt{d, run_of};
+    launch_interaction(dt, st, rt, beg, end, mats, nmat - nart, nbr, variant, false, s);
+    if (nart > 0)
+        launch_interaction(dt, st, rt, beg, end, IndexMap{mats.idx + (nmat - nart), 0}, nart, nbr, variant, true, s);
 }
 
 void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, const DevIni& ini,

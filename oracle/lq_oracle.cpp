@@ -418,12 +418,23 @@ void lqo_interaction(const lq_params* par, const lqo_state* st, lqo_rates* rt, c
     const double i_h = par->i_h, cg = par->kernel_gradient_normalization_par, m = par->m, h = par->h, eta = par->eta;
     const double alpha_cs =
         par->viscosity_type == 1 ? (double)par->artificial_viscosity_alpha * (double)par->cs : 0.0;
+    // Artificial pressure (artificial-pressure = 1).  Specification decision:
+    // the reference derives W(dr) for it (src/loquat.cu:133-138) but the term
+    // lives in the unshipped interaction body.  Monaghan's form (J. Comput.
+    // Phys. 159, 2000) with n = 4: R_i = eps * max(-p_i, 0) / rho_i^2 (p =
+    // pospres.w, tension negative), f_ij = W(r_ij) / W(dr), and
+    // (R_i + R_j) f_ij^4 is subtracted from the diagonal of the momentum
+    // sum like a pressure.
+    const bool art = par->artificial_pressure == 1;
+    const double eps = art ? (double)par->artificial_pressure_coefficient : 0.0;
+    const double cw = par->kernel_normalization_par, inv_wdr = art ? 1.0 / (double)par->kernel_dr : 0.0;
     const int n = par->np;
 #pragma omp parallel for schedule(dynamic, 256)
     for (int i = par->npb; i < n; ++i) {
         const float xf = (float)st->x[i], yf = (float)st->y[i], zf = (float)st->z[i];
         const double ri = st->rho[i];
         const T6 Ti = mul(load6(st->s, i), 1.0 / (ri * ri));
+        const double Ri = eps * std::max(-st->p[i], 0.0) / (ri * ri);
         double ax = 0, ay = 0, az = 0, drho = 0, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         int count = 0;
         const int cix = cell_coord(xf, i_hf), ciy = cell_coord(yf, i_hf), ciz = cell_coord(zf, i_hf);
@@ -451,6 +462,11 @@ void lqo_interaction(const lq_params* par, const lqo_state* st, lqo_rates* rt, c
                     if (vdx < 0.0) {
                         const double mu = h * vdx / (r2 + eta);
                         pij = -alpha_cs * mu / (0.5 * (ri + rj));
+                    }
+                    if (art) {
+                        const double Rj = eps * std::max(-st->p[j], 0.0) / (rj * rj);
+                        const double f = cw * t * t * t * t * (q + 0.5) * inv_wdr;
+                        pij += (Ri + Rj) * (f * f) * (f * f);
                     }
                     const T6 S = add(Ti, mul(load6(st->s, j), 1.0 / (rj * rj)));
                     const double sxx = S.xx - pij, syy = S.yy - pij, szz = S.zz - pij;

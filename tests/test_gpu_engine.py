@@ -76,6 +76,45 @@ def test_engine_vs_oracle_steps(gpu, model):
 
 
 @pytest.mark.gpu
+def test_engine_artificial_pressure_vs_oracle(gpu):
+    """20 steps with artificial-pressure = 1, beside a run without it in
+    the same batch: a cohesive Drucker-Prager column whose top layers start
+    in isotropic tension."""
+    case = gentle_case(1, seed=31)
+    case.par.model_par_5 = 4.0e3          # DP-kcohesion: tension is admissible
+    top = case.posd[:, 2] > np.quantile(case.posd[case.par.npb:, 2], 0.8)
+    top[:case.par.npb] = False
+    case.str[top, :3] += 1.5e3
+    case.pospres[:, 3] = -case.str[:, :3].sum(1) / 3.0
+    case.par.artificial_pressure = 1
+    case.par.artificial_pressure_coefficient = 0.3
+    other = gentle_case(1, seed=32)
+    nsteps = 20
+    e, (rid, oid) = _engine([case, other], nsteps)
+    e.step(nsteps)
+    got = e.download(rid)
+    p = case.derived()
+    st = O.OState.from_case(case)
+    O.initial_sort(p, st)
+    O.run(p, case.vw, st, nsteps)
+    ref = O.unpermute(st)
+    m = slice(p.npb, None)
+    x0 = case.posd
+    for a, k in enumerate(("x", "y", "z")):
+        assert rel_err(got["posd"][m, a] - x0[m, a], getattr(ref, k)[m] - x0[m, a]) < 1e-4, k
+    for a, k in enumerate(("vx", "vy", "vz")):
+        assert rel_err(got["velrhop"][m, a], getattr(ref, k)[m]) < 1e-4, k
+    for c in range(6):
+        assert rel_err(got["str"][m, c], ref.s[c, m]) < 1e-4, c
+    # the run without the term is unaffected by its batch neighbour
+    e2, (rid2,) = _engine([other], nsteps)
+    e2.step(nsteps)
+    assert _same(e.download(oid), e2.download(rid2))
+    e.close()
+    e2.close()
+
+
+@pytest.mark.gpu
 def test_batch_equals_alone(gpu):
     runs = cases.ensemble(nruns=5, seed=3, nsteps=30, npm_lo=500, npm_hi=4000)
     alone = []

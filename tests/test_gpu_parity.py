@@ -181,6 +181,37 @@ def test_interaction_parity(gpu, variant, which):
     g.close()
 
 
+@pytest.mark.gpu
+def test_interaction_parity_artificial_pressure(gpu):
+    """artificial-pressure = 1 with a third of the particles in tension."""
+    dev = _dev()
+    import torch
+    base = cases.perturb(cases.small_column_3d(seed=15), seed=16)
+    base.par.artificial_pressure = 1
+    base.par.artificial_pressure_coefficient = 0.3
+    rng = np.random.default_rng(17)
+    base.pospres[:, 3] = rng.uniform(-1e4, 2e4, base.np).astype(np.float32)
+    p, ds, ost = sorted_inputs(base)
+    assert p.kernel_dr > 0.0 and (ost.p[p.npb:] < 0).sum() > 100
+    g, beg, end, ref = _grid_tables(p, ds, ost)
+    rt = dev.DeviceRates(ds.n)
+    nbr = torch.full((ds.n,), -1, dtype=torch.int32, device="cuda")
+    dev.interaction(p, ds, rt, beg, end, nbr)
+    ort, onbr = O.interaction(p, ost, ref["cell_beg"], ref["cell_end"])
+    h = rt.host()
+    npb = p.npb
+    m = slice(npb, None)
+    assert np.array_equal(nbr.cpu().numpy()[npb:], onbr[npb:])
+    for a, k in enumerate(("ax", "ay", "az", "drho")):
+        assert rel_err(h["accad"][m, a], getattr(ort, k)[m]) < TOL, k
+    # the term is active: without it the accelerations differ well beyond TOL
+    off = p.copy()
+    off.artificial_pressure = 0
+    o0, _ = O.interaction(off, ost, ref["cell_beg"], ref["cell_end"])
+    assert rel_err(o0.az[m], ort.az[m]) > 100 * TOL
+    g.close()
+
+
 def _rates_from_oracle(ort: O.ORates, n: int):
     dev = _dev()
     import torch

@@ -202,6 +202,77 @@ def test_viscosity_dissipates_approach_only():
 
 
 # -------------------------------------------------------------- Adami
+def _with_artificial_pressure(case, coef: float):
+    case.par.artificial_pressure = 1
+    case.par.artificial_pressure_coefficient = coef
+    return case
+
+
+def test_artificial_pressure_off_without_tension():
+    """R = eps max(-p, 0) / rho^2: with no particle in tension (p >= 0) the
+    term adds exactly zero, so the rates equal those with it switched off."""
+    case = cases.perturb(lattice_case(walls=False), seed=5, v_amp=0.3)
+    rng = np.random.default_rng(6)
+    case.pospres[:, 3] = rng.uniform(0.0, 5e3, case.np).astype(np.float32)
+    p0, st0, g0 = sorted_oracle(case)
+    r0, _ = O.interaction(p0, st0, g0["cell_beg"], g0["cell_end"])
+    p1, st1, g1 = sorted_oracle(_with_artificial_pressure(case, 0.3))
+    assert p1.kernel_dr > 0.0
+    r1, _ = O.interaction(p1, st1, g1["cell_beg"], g1["cell_end"])
+    for k in ("ax", "ay", "az", "drho", "wx"):
+        assert np.array_equal(getattr(r0, k), getattr(r1, k)), k
+
+
+def test_artificial_pressure_pair_closed_form():
+    """Two particles at rest, zero deviatoric stress, particle 0 in tension:
+    the only force is Monaghan's term, a_0 = -m (R_0 + R_1) f^4 grad_0 W with
+    R = eps |p| / rho^2 and f = W(r)/W(dr), W the Wendland C2 kernel
+    21/(16 pi h^3) (1 - q/2)^4 (2q + 1).  It pushes the pair apart."""
+    c = _with_artificial_pressure(_two_particles(0.0), 0.25)
+    c.str[:] = 0.0
+    c.pospres[:, 3] = (-4.0e4, 0.0)
+    p = c.derived()
+    st = O.OState.from_case(c)
+    g = O.grid_build(p, st)
+    O.permute(g["part_idx"], st)
+    g = O.grid_build(p, st)
+    rt, nbr = O.interaction(p, st, g["cell_beg"], g["cell_end"])
+    assert list(nbr) == [1, 1]
+    i0 = int(np.nonzero(st.orig == 0)[0][0])
+    i1 = 1 - i0
+    h = float(p.h)
+    r = abs(st.x[i1] - st.x[i0])
+    q, qd = r / h, float(p.dr) / h
+    shape = lambda q_: (1.0 - 0.5 * q_) ** 4 * (2.0 * q_ + 1.0)
+    f = shape(q) / shape(qd)
+    alpha = 21.0 / (16.0 * math.pi * h ** 3)
+    dwdr = -5.0 * alpha * q * (1.0 - 0.5 * q) ** 3 / h          # dW/dr
+    rho = float(st.rho[i0])
+    R0 = 0.25 * 4.0e4 / rho ** 2
+    grad0 = dwdr * np.sign(st.x[i0] - st.x[i1])                 # x component of grad_0 W
+    expect = -float(p.m) * R0 * f ** 4 * grad0
+    assert rt.ax[i0] - p.acc_x == pytest.approx(expect, rel=2e-6)
+    assert rt.ax[i1] - p.acc_x == pytest.approx(-expect, rel=2e-6)
+    assert np.sign(rt.ax[i0]) == np.sign(st.x[i0] - st.x[i1])   # repulsive
+    assert rt.ay[i0] == pytest.approx(p.acc_y) and rt.drho[i0] == 0.0
+
+
+def test_artificial_pressure_conserves_momentum():
+    case = _with_artificial_pressure(cases.perturb(lattice_case(walls=False), seed=3, v_amp=0.5), 0.3)
+    rng = np.random.default_rng(4)
+    case.pospres[:, 3] = rng.uniform(-2e4, 2e4, case.np).astype(np.float32)
+    p, st, g = sorted_oracle(case)
+    rt, _ = O.interaction(p, st, g["cell_beg"], g["cell_end"])
+    m = float(p.m)
+    f = np.stack([rt.ax - p.acc_x, rt.ay - p.acc_y, rt.az - p.acc_z], 1) * m
+    assert np.abs(f.sum(0)).max() < 1e-10 * np.abs(f).sum(0).max()
+    off = case.par.copy()
+    off.artificial_pressure = 0
+    p0 = abi.derive(off)
+    r0, _ = O.interaction(p0, st, g["cell_beg"], g["cell_end"])
+    assert np.abs(rt.ax - r0.ax).max() > 1e-2 * np.abs(r0.ax - p.acc_x).max()   # the term is active
+
+
 def test_adami_extrapolates_linear_hydrostatic_field_exactly():
     case = cases.small_column_3d(seed=12, jitter=0.1)
     p, st, g = sorted_oracle(case)
