@@ -33,8 +33,8 @@ BYTES_WRITE_PER_MATERIAL = 52  # a (12) + drho (4) + spin (12) + strain rate (24
 BYTES_PER_CELL = 8             # cell_beg + cell_end
 WORKLOAD_ID = "column_3d dx=5mm npm=1024000"  # the key of this workload in profiles/*.json
 KERNEL_NAMES = {0: "k_interaction_direct", 1: "k_interaction_lds", 2: "k_interaction_unrolled",
-                3: "k_interaction_v3<DescTable, 1, false, 1, 4>", 4: "k_interaction_v3<DescTable, 2, false, 1, 4>",
-                5: "k_interaction_v3<DescTable, 1, true, 1, 4>", 6: "k_interaction_v3<DescTable, 2, true, 1, 4>"}
+                3: "k_interaction_v3<DescTable, 1, false, false>", 4: "k_interaction_v3<DescTable, 2, false, false>",
+                5: "k_interaction_v3<DescTable, 1, true, false>", 6: "k_interaction_v3<DescTable, 2, true, false>"}
 
 
 def log(msg: str) -> None:

@@ -365,6 +365,9 @@ __global__ __launch_bounds__(256) void k_interaction_lds(DP dp, DevState st, Dev
 // between column segments, so the loop carries no accumulator registers.
 constexpr int IC2_CAP = 32;
 constexpr int IC2_U = 4;
+// List length of variants 3 to 6: 4 x 40 x 64 x 4 B = 40 KB per 256-thread
+// workgroup, so four workgroups (the register limit) fill the CU's 160 KB.
+constexpr int IC3_CAP = 40;
 
 template <class DP>
 __global__ __launch_bounds__(256) void k_interaction_unrolled(DP dp, DevState st, DevRates rt,
@@ -547,7 +550,8 @@ __device__ __forceinline__ void pair_eval3(const IPart& pi, uint32_t j, const De
 
 // Wave-cooperative neighbour scan shared by the list-based kernels.  For a
 // lane with `active` set it walks the z-merged (x, y) columns of cell layers
-// [layer_lo, layer_hi] around cell (cix, ciy, ciz), tests candidates four at a
+// [layer_lo, layer_hi] around cell (cix, ciy, ciz) -- (x, y) major, layers
+// inside -- tests candidates four at a
 // time with the FP32 predicate (branch-free, four loads in flight), appends
 // hits to the lane's LDS list `my`, and calls eval(j, true) for every listed
 // j, in candidate order, whenever some lane's list is nearly full and at the
@@ -560,8 +564,11 @@ __device__ __forceinline__ void pair_eval3(const IPart& pi, uint32_t j, const De
 // (x, y) distance already exceeds the support is skipped, and when cells are
 // sorted by z-bin (zbits > 0) each column is read only over the z-window
 // |dz| <= sqrt(R^2 - dxy_min^2) widened by one bin: blocks of four lying wholly
-// below it are stepped over with one load, and the scan stops at the first
-// block whose last particle lies above it.  R is the support radius inflated
+// below it are stepped over (four blocks per load round), and the scan stops
+// at the first block whose last particle lies above it.  The kernel is bound
+// by the latency of dependent load rounds, so each column's set-up issues its
+// loads together: both layers' cell-table entries in one round, the z-window
+// start in (usually) one more.  R is the support radius inflated
 // by 1e-3 and dxy_min is shrunk by 1e-3 of a cell, far beyond the FP32
 // rounding of the coordinates, so every skipped candidate has r2 > r2_cut.
 struct ScanGeom {
@@ -599,12 +606,13 @@ __device__ __forceinline__ ScanGeom make_geom(const RunDesc& d, float4 ps) {
 //
 // B = 2 evaluates the list two entries at a time, eval(j0, act0, j1, act1),
 // so that both gathers are in flight before either pair is computed.
-template <int B, bool PADDED, int U = IC2_U, class Eval>
+template <int B, bool PADDED, int CAP = IC2_CAP, class Eval>
 __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer_hi, const ScanGeom& g, float4 ps,
                                             float r2cut, const int32_t* __restrict__ beg,
                                             const int32_t* __restrict__ end, const float4* __restrict__ pp,
                                             int32_t (*my)[64], int lane, uint32_t dummy, Eval&& eval) {
     static_assert(B == 1 || B == 2, "batch of 1 or 2");
+    constexpr int U = IC2_U;  // candidates per load round (SCAN_PAD covers a block past a range's end)
     int count = 0, cnt = 0;
     auto flush = [&]() {
         // bottom-tested loop: the accumulators flow straight from one
@@ -620,7 +628,7 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
                 } else {
                     const bool a0 = e < cnt, a1 = e + 1 < cnt;
                     const uint32_t j0 = (uint32_t)my[e][lane];
-                    const uint32_t j1 = (uint32_t)my[min(e + 1, IC2_CAP - 1)][lane];
+                    const uint32_t j1 = (uint32_t)my[min(e + 1, CAP - 1)][lane];
                     eval(a0 ? j0 : dummy, a0, a1 ? j1 : dummy, a1);
                 }
                 e += B;
@@ -631,42 +639,90 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
     };
     const float* __restrict__ pz = reinterpret_cast<const float*>(pp) + 2;  // pz[4 j] = z of particle j
     const int czlo = max(g.ciz - 1, 0), czhi = min(g.ciz + 1, g.nz - 1);
-    const int ncol = 9 * (layer_hi - layer_lo + 1);
+    const int nl = layer_hi - layer_lo + 1;
+    const int ncol = 9 * nl;
     const float margin = 1e-3f * g.cw;
+    // [b0, e0) of the column's cells czlo..czhi starting at cell id c0 (b0 =
+    // -1 when all are empty).  The (up to) three cells' six table entries are
+    // loaded together; particles of consecutive cells are consecutive, so the
+    // range runs from the first non-empty cell's begin to the last one's end.
+    auto col_range = [&](int c0, int& b0, int& e0) {
+        const int ca = c0 + czlo, cb = c0 + min(czlo + 1, czhi), cc = c0 + czhi;
+        const int ba = beg[ca], bb = beg[cb], bc = beg[cc];
+        const int ea = end[ca], eb = end[cb], ec = end[cc];
+        b0 = ba != -1 ? ba : (bb != -1 ? bb : bc);
+        e0 = bc != -1 ? ec : (bb != -1 ? eb : ea);
+    };
+    // First candidate of the non-empty range [b0, e0) that may lie in the
+    // z-window.  A block of U whose last particle is below zlo lies wholly
+    // below the window, and so does everything before it (z-bins ascend
+    // along the column); four blocks are tested per round, their last z
+    // loaded together.
+    auto skip_below = [&](int b0, int e0, float zlo) {
+        int j = b0;
+        while (true) {
+            int adv = 0;
+#pragma unroll
+            for (int m = 1; m <= 4; ++m) {
+                const int last = j + m * U - 1;
+                const float zm = ldo(pz, 4u * (uint32_t)min(last, e0 - 1));
+                if (last < e0 && zm < zlo) adv = m * U;
+            }
+            j += adv;
+            if (adv < 4 * U) break;
+        }
+        return j;
+    };
+    // Columns are visited (x, y) major with the layers inside, so that one
+    // set-up (geometry, table loads, z-window) serves both layers: the
+    // second layer's range is prepared with the first and kept meanwhile.
+    int jn = 0, jen = 0;
+    float zhin = INFINITY;
     for (int col = 0; col < ncol; ++col) {
-        const int layer = layer_lo + col / 9, ox = (col % 9) / 3 - 1, oy = col % 3 - 1;
+        const int xy = col / nl, layer = layer_lo + col % nl;
+        const int ox = xy / 3 - 1, oy = xy % 3 - 1;
         int j = 0, je = 0;
         float zhi = INFINITY;
-        if (active && czlo <= czhi) {
-            const int cx = g.cix + ox, cy = g.ciy + oy;
-            const float ex = ox == 0 ? 0.0f : fmaxf((ox < 0 ? g.u : 1.0f - g.u) * g.cw - margin, 0.0f);
-            const float ey = oy == 0 ? 0.0f : fmaxf((oy < 0 ? g.v : 1.0f - g.v) * g.cw - margin, 0.0f);
-            const float hz2 = g.rw2 - (ex * ex + ey * ey);
-            if (hz2 > 0.0f && cx >= 0 && cx < g.nx && cy >= 0 && cy < g.ny) {
-                const int c0 = g.cbase + layer * g.nol + (cx * g.ny + cy) * g.nz;
-                int b0 = -1, e0 = -1;
-                for (int cz = czlo; cz <= czhi; ++cz) {
-                    const int b = beg[c0 + cz];
-                    if (b != -1) {
-                        if (b0 < 0) b0 = b;
-                        e0 = end[c0 + cz];
-                    }
-                }
-                if (b0 >= 0) {
-                    j = b0;
-                    je = e0;
+        if (layer != layer_lo) {
+            j = jn;
+            je = jen;
+            zhi = zhin;
+        } else {
+            jn = jen = 0;
+            zhin = INFINITY;
+            if (active && czlo <= czhi) {
+                const int cx = g.cix + ox, cy = g.ciy + oy;
+                const float ex = ox == 0 ? 0.0f : fmaxf((ox < 0 ? g.u : 1.0f - g.u) * g.cw - margin, 0.0f);
+                const float ey = oy == 0 ? 0.0f : fmaxf((oy < 0 ? g.v : 1.0f - g.v) * g.cw - margin, 0.0f);
+                const float hz2 = g.rw2 - (ex * ex + ey * ey);
+                if (hz2 > 0.0f && cx >= 0 && cx < g.nx && cy >= 0 && cy < g.ny) {
+                    const int c0 = g.cbase + layer * g.nol + (cx * g.ny + cy) * g.nz;
+                    int b0, e0, b1 = -1, e1 = -1;
+                    col_range(c0, b0, e0);
+                    if (nl == 2) col_range(c0 + g.nol, b1, e1);
+                    float zlo = -INFINITY;
                     if (g.zbits) {
                         const float hz = sqrtf(hz2);
-                        const float zlo = g.z - hz - g.binw;
+                        zlo = g.z - hz - g.binw;
                         zhi = g.z + hz + g.binw;
-                        while (j + U <= je && ldo(pz, 4u * (uint32_t)(j + U - 1)) < zlo) j += U;
+                    }
+                    if (b1 >= 0) {
+                        jn = g.zbits ? skip_below(b1, e1, zlo) : b1;
+                        jen = e1;
+                        zhin = zhi;
+                    }
+                    if (b0 >= 0) {
+                        j = g.zbits ? skip_below(b0, e0, zlo) : b0;
+                        je = e0;
+                    } else {
+                        zhi = INFINITY;
                     }
                 }
             }
         }
         const uint32_t jlast = (uint32_t)max(je - 1, 0);
         while (true) {
-            while (__ballot(j < je) && !__ballot(cnt > IC2_CAP - U)) {
+            while (__ballot(j < je) && !__ballot(cnt > CAP - U)) {
                 // a lane whose column is done issues no loads (module calls
                 // keep every lane in step: their loads are clamped instead)
                 if (PADDED && j >= je) continue;
@@ -702,7 +758,7 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
             flush();
         }
         // flush between columns when another column might not fit
-        if (__ballot(cnt > IC2_CAP - (U == 4 ? 3 * U : 2 * U))) flush();
+        if (__ballot(cnt > CAP - 3 * U)) flush();
     }
     flush();
     return count;
@@ -715,13 +771,13 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
 // zeroes the kernel gradient and v_i - v_i the velocity terms), so it changes
 // no sum, and one is taken off the neighbour count.  Variant 4 (B = 2)
 // evaluates the lists two pairs at a time; same pairs, same order.
-template <class DP, int B, bool MASKED = false, int MINW = 1, int U = IC2_U, bool ART = false>
-__global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(MINW))) void k_interaction_v3(
+template <class DP, int B, bool MASKED = false, bool ART = false>
+__global__ __launch_bounds__(256) void k_interaction_v3(
     DP dp, DevState st, DevRates rt,
                                                         const int32_t* __restrict__ beg,
                                                         const int32_t* __restrict__ end, IndexMap mats, int n,
                                                         int32_t* __restrict__ nbr) {
-    __shared__ int32_t lists[4][IC2_CAP][64];
+    __shared__ int32_t lists[4][IC3_CAP][64];
     const int t = xcd_thread();
     const int k = t < n ? mats(t) : -1;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -765,11 +821,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_
         }
     };
     if constexpr (B == 1) {
-        count = scan_compact<1, DP::padded, U>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+        count = scan_compact<1, DP::padded, IC3_CAP>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
                                             mat ? (uint32_t)k : 0u,
                                             [&](uint32_t j, bool act) { pair_compute<ART>(pi, load(j, act), c, a); });
     } else {
-        count = scan_compact<2, DP::padded, U>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+        count = scan_compact<2, DP::padded, IC3_CAP>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
                                             mat ? (uint32_t)k : 0u,
                                             [&](uint32_t j0, bool a0, uint32_t j1, bool a1) {
                                                 const PairIn q0 = load(j0, a0);
@@ -1343,8 +1399,8 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
                                hipStream_t s) {
     if (n <= 0) return;
     const int nb = blocks_for(n, 256);
-    if (art)  // artificial pressure: one kernel (variant 6's) carries the extra term
-        k_interaction_v3<DP, 2, true, 1, IC2_U, true><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+    if (art)  // artificial pressure: one kernel (variant 6's shape) carries the extra term
+        k_interaction_v3<DP, 2, true, true><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else if (variant == 0)
         k_interaction_direct<DP><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else if (variant == 1)
