@@ -221,6 +221,27 @@ lq_status lq_particle_interaction(const lq_params* par, const lq_state* st, cons
                                   const int32_t* cell_beg, const int32_t* cell_end,
                                   int32_t* nbr_count, lq_stream stream);
 
+/* Stale-table variants (no reference counterpart; the engine's corrector
+ * half uses them instead of a second neighbour search).  cell_beg / cell_end
+ * come from a grid built when the particles were elsewhere: the state st is
+ * in that build's sorted order, (x_then, y_then, z_then) are the material
+ * particles' FP64 positions at build time (walls do not move), and no
+ * particle has moved more than disp since, 0 <= 2 disp <= 0.9 * 2h
+ * (LQ_ERR_INVALID_ARG otherwise).  Results use the neighbour sets a fresh
+ * grid of the current positions gives (the 27-cell stencil of each
+ * particle's current cell, q < 2): the scans reach disp further and test
+ * every hit that could differ against its current cell.  Only the order of
+ * the FP32 sums differs from the fresh-grid call.
+ * lq_adami_boundary_stale: nbr_count (nullable, int32[np]) receives each
+ * wall's material neighbour count. */
+lq_status lq_particle_interaction_stale(const lq_params* par, const lq_state* st, const lq_rates* rates,
+                                        const double* x_then, const double* y_then, const double* z_then,
+                                        float disp, const int32_t* cell_beg, const int32_t* cell_end,
+                                        int32_t* nbr_count, lq_stream stream);
+lq_status lq_adami_boundary_stale(const lq_params* par, const lq_state* st, float disp,
+                                  const int32_t* cell_beg, const int32_t* cell_end, int32_t* nbr_count,
+                                  lq_stream stream);
+
 /* Predictor (src/predictor_corrector.cu:221-253): copies the material state
  * into ini, then advances it half a step with the constitutive update.
  * vw is read only when constitutive_model == 3 (may be NULL otherwise). */

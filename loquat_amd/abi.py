@@ -117,6 +117,9 @@ _PROTOS = {
                               C.POINTER(LqIni), vp]),
     "lq_grid_arrays": (i32, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]),
     "lq_adami_boundary": (i32, [C.POINTER(LqParams), C.POINTER(LqState), vp, vp, vp]),
+    "lq_adami_boundary_stale": (i32, [C.POINTER(LqParams), C.POINTER(LqState), C.c_float, vp, vp, vp, vp]),
+    "lq_particle_interaction_stale": (i32, [C.POINTER(LqParams), C.POINTER(LqState), C.POINTER(LqRates), vp, vp,
+                                            vp, C.c_float, vp, vp, vp, vp]),
     "lq_particle_interaction": (i32, [C.POINTER(LqParams), C.POINTER(LqState), C.POINTER(LqRates), vp, vp, vp,
                                       vp]),
     "lq_predictor": (i32, [C.POINTER(LqParams), C.POINTER(LqVw), C.POINTER(LqState), C.POINTER(LqIni),

@@ -72,6 +72,31 @@ struct IndexMap {
     __device__ __forceinline__ int operator()(int t) const { return idx ? idx[t] : base + t; }
 };
 
+// A cell table built before the particles last moved ("stale"): the FP64
+// positions of the particles being updated when it was built, and a bound on
+// how far any particle of a run has moved since -- per run as float bits
+// (dmax[run * DMAX_SLOTS]), or one value (disp) when dmax is null.
+// nout (nullable, per run): nonzero when some particle of the run lies
+// outside the grid now (null: assume so).
+struct StaleIn {
+    const double* xo;
+    const double* yo;
+    const double* zo;
+    const uint32_t* dmax;
+    float disp;
+    const uint32_t* nout;
+};
+
+// Per-run displacement maxima are accumulated in DMAX_SLOTS slots per run
+// (workgroup b uses slot b % DMAX_SLOTS, so that atomics do not pile up on
+// one address) and folded into slot 0 before they are read.
+constexpr int DMAX_SLOTS = 64;
+
+// Largest displacement a stale table tolerates, as a fraction of the cell
+// width: 2 disp <= STALE_LIMIT * (2 h).  A run that moves further in half a
+// step has violated any CFL bound and is stopped as diverging.
+constexpr float STALE_LIMIT = 0.9f;
+
 struct DevState {
     double* xd;
     double* yd;

@@ -246,7 +246,8 @@ extern "C" lq_status lq_adami_boundary(const lq_params* par, const lq_state* st,
         return LQ_ERR_INVALID_ARG;
     }
     if (par->npb == 0) return LQ_OK;  // src/adami_boundary.cu:135
-    launch_adami_one(one_desc(par, nullptr), to_dev(*st), cell_beg, cell_end, g_variant, (hipStream_t)stream);
+    launch_adami_one(one_desc(par, nullptr), to_dev(*st), cell_beg, cell_end, g_variant, nullptr, nullptr,
+                     (hipStream_t)stream);
     return last_launch("lq_adami_boundary");
 }
 
@@ -260,8 +261,49 @@ extern "C" lq_status lq_particle_interaction(const lq_params* par, const lq_stat
     }
     if (par->npm == 0) return LQ_OK;
     launch_interaction_one(one_desc(par, nullptr), to_dev(*st), to_dev(*rates), cell_beg, cell_end, nbr_count,
-                           g_variant, (hipStream_t)stream);
+                           g_variant, nullptr, (hipStream_t)stream);
     return last_launch("lq_particle_interaction");
+}
+
+static lq_status check_disp(const lq_params* par, float disp, const char* who) {
+    if (!(disp >= 0.0f) || !(2.0f * disp <= STALE_LIMIT * (2.0f / par->i_h))) {
+        set_error(std::string(who) + ": displacement must lie in [0, 0.45 * 2h]");
+        return LQ_ERR_INVALID_ARG;
+    }
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_particle_interaction_stale(const lq_params* par, const lq_state* st, const lq_rates* rates,
+                                                   const double* x_then, const double* y_then, const double* z_then,
+                                                   float disp, const int32_t* cell_beg, const int32_t* cell_end,
+                                                   int32_t* nbr_count, lq_stream stream) {
+    LQ_TRY(check_params(par));
+    if (!state_ok(st) || !rates_ok(rates) || !x_then || !y_then || !z_then || !cell_beg || !cell_end) {
+        set_error("lq_particle_interaction_stale: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    LQ_TRY(check_disp(par, disp, "lq_particle_interaction_stale"));
+    if (par->npm == 0) return LQ_OK;
+    const StaleIn stale{x_then, y_then, z_then, nullptr, disp, nullptr};
+    launch_interaction_one(one_desc(par, nullptr), to_dev(*st), to_dev(*rates), cell_beg, cell_end, nbr_count,
+                           g_variant, &stale, (hipStream_t)stream);
+    return last_launch("lq_particle_interaction_stale");
+}
+
+extern "C" lq_status lq_adami_boundary_stale(const lq_params* par, const lq_state* st, float disp,
+                                             const int32_t* cell_beg, const int32_t* cell_end, int32_t* nbr_count,
+                                             lq_stream stream) {
+    LQ_TRY(check_params(par));
+    if (!state_ok(st) || !cell_beg || !cell_end) {
+        set_error("lq_adami_boundary_stale: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    LQ_TRY(check_disp(par, disp, "lq_adami_boundary_stale"));
+    if (par->npb == 0) return LQ_OK;
+    const StaleIn stale{nullptr, nullptr, nullptr, nullptr, disp, nullptr};
+    launch_adami_one(one_desc(par, nullptr), to_dev(*st), cell_beg, cell_end, g_variant, &stale, nbr_count,
+                     (hipStream_t)stream);
+    return last_launch("lq_adami_boundary_stale");
 }
 
 static lq_status integrate(bool corr, const lq_params* par, const lq_vw_params* vw, const lq_state* st,
@@ -325,6 +367,8 @@ struct Buffers {
     RunDesc* desc;
     int32_t* fail;  // per-run flag, raised on the device by the corrector
     uint32_t* dtred;  // per-run |v|max, |a|max of the dynamic time step (2 per run, kept zero between steps)
+    uint32_t* dmax;   // per-run largest displacement of the predictor half (float bits, DMAX_SLOTS per run)
+    uint32_t* nout;   // per run: some particle ended the predictor half outside the grid
     int32_t* mat_idx;   // particle index of the t-th material particle (all runs)
     int32_t* wall_idx;  // particle index of the t-th wall particle (all runs)
     GridBuffers grid;
@@ -404,43 +448,55 @@ void enqueue_build(lq_engine* e, const DevState& st, hipStream_t s) {
     launch_grid_build_table(b.desc, b.run_of, st.pospres, n, e->ncell_total, zbits_eff(e), b.grid, s);
 }
 
-// One full step for every resident run, starting from buffer e->cur (the
-// caller flips cur afterwards when shepard is set).
-void enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
+// One full step for every resident run, starting from buffer `cur`; returns
+// the buffer that holds the state afterwards (step_out).
+//
+// The neighbour search runs once per step, in the predictor half (sort,
+// permute, cell table), and again before a Shepard filter.  The corrector
+// half scans the predictor half's table: particles have moved by at most the
+// displacement the predictor records per run (hdt |v|, a small fraction of a
+// cell), and the stale scans widen their reach by it and test each hit
+// against its current cell, so every neighbour set is the one a fresh sort
+// gives (the reference sorts again, src/loquat.cu:362-366).  The ini set is
+// in the order of the state, so the corrector reads it directly.
+int enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
     Buffers& b = e->b;
     hipStream_t s = e->stream;
     const int n = (int)e->np_total;
     const int nr = (int)e->runs.size();
     const int var = e->variant >= 0 ? e->variant : g_variant;
-    launch_step_begin(b.desc, nr, s);
+    launch_step_begin(b.desc, nr, b.dmax, b.nout, s);
     if (e->any_dynamic)
         launch_dynamic_dt(b.desc, b.run_of, b.S[cur], b.R, IndexMap{b.mat_idx, 0}, (int)e->nmat_total, nr, b.dtred, s);
     int c = cur;
+    const IndexMap walls{b.wall_idx, 0}, mats{b.mat_idx, 0};
+    const int nw = (int)e->nwall_total, nm = (int)e->nmat_total;
     for (int half = 0; half < 2; ++half) {
         const bool corr = half == 1;
-        {
+        if (!corr) {
             StageTimer t(prof, s, PG_GRID);
             enqueue_build(e, b.S[c], s);
-            // The ini set stays in the predictor-half order: the corrector
-            // reads it through this build's permutation.
             launch_permute(b.grid.vals[b.grid.sorted_slot], n, b.S[c], b.S[1 - c], nullptr, nullptr, nullptr, s);
+            c = 1 - c;
         }
-        c = 1 - c;
-        const IndexMap walls{b.wall_idx, 0}, mats{b.mat_idx, 0};
-        const int nw = (int)e->nwall_total, nm = (int)e->nmat_total;
+        // corrector half: the table and the positions it was built from (the
+        // ini set holds the material particles' positions of that time)
+        const StaleIn stale{b.I.xd, b.I.yd, b.I.zd, b.dmax, 0.0f, b.nout};
+        const StaleIn* sp = corr ? &stale : nullptr;
         {
             StageTimer t(prof, s, PG_ADAMI);
-            launch_adami_table(b.desc, b.run_of, b.S[c], b.grid.cell_beg, b.grid.cell_end, walls, nw, var, s);
+            launch_adami_table(b.desc, b.run_of, b.S[c], b.grid.cell_beg, b.grid.cell_end, walls, nw, var, sp, s);
         }
         {
             StageTimer t(prof, s, PG_INTER);
             launch_interaction_table(b.desc, b.run_of, b.S[c], b.R, b.grid.cell_beg, b.grid.cell_end, mats, nm,
-                                     (int)e->nmat_art, nullptr, var, s);
+                                     (int)e->nmat_art, nullptr, var, sp, s);
         }
         {
             StageTimer t(prof, s, PG_INTEG);
-            launch_integrate_table(corr, b.desc, b.run_of, b.S[c], b.I, b.R, mats, nm,
-                                   corr ? b.grid.vals[b.grid.sorted_slot] : nullptr, b.fail, s);
+            launch_integrate_table(corr, b.desc, b.run_of, b.S[c], b.I, b.R, mats, nm, nullptr, b.fail, b.dmax, b.nout,
+                                   s);
+            if (!corr) launch_dmax_fold(b.dmax, nr, s);
         }
     }
     if (shepard) {
@@ -455,7 +511,12 @@ void enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
                              s);
     }
     launch_step_end(b.desc, nr, b.fail, s);
+    return c;
 }
+
+// Buffer holding the state after a step from `cur` (one permute, plus one
+// for a Shepard filter).
+inline int step_out(int cur, bool shepard) { return shepard ? cur : 1 - cur; }
 
 // Host mirror of k_step_begin's Shepard test over all active runs.
 bool any_shepard_due(const lq_engine* e) {
@@ -619,7 +680,8 @@ extern "C" lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t
          dmalloc(b.R.acestn, np_cap, e->owned) && dmalloc(b.T, np_cap, e->owned) &&
          dmalloc(b.run_of, np_cap, e->owned) && dmalloc(b.desc, max_runs, e->owned) &&
          dmalloc(b.fail, max_runs, e->owned) && dmalloc(b.mat_idx, np_cap, e->owned) &&
-         dmalloc(b.wall_idx, np_cap, e->owned) && dmalloc(b.dtred, 2 * (int64_t)max_runs, e->owned);
+         dmalloc(b.wall_idx, np_cap, e->owned) && dmalloc(b.dtred, 2 * (int64_t)max_runs, e->owned) &&
+         dmalloc(b.dmax, (int64_t)max_runs * DMAX_SLOTS, e->owned) && dmalloc(b.nout, max_runs, e->owned);
     ok = ok && hipMemset(b.dtred, 0, 2 * (size_t)max_runs * sizeof(uint32_t)) == hipSuccess;
     if (!ok || alloc_grid(b.grid, np_cap, ncell_cap) != LQ_OK) {
         for (void* p : e->owned) (void)hipFree(p);
@@ -819,7 +881,7 @@ extern "C" lq_status lq_engine_step(lq_engine* e, int32_t nsteps) {
             if (!g) {
                 hipGraph_t graph = nullptr;
                 LQ_HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-                enqueue_step(e, e->cur, shep, nullptr);
+                (void)enqueue_step(e, e->cur, shep, nullptr);
                 LQ_HIP_TRY(hipStreamEndCapture(e->stream, &graph));
                 hipError_t err = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(graph);
@@ -831,10 +893,10 @@ extern "C" lq_status lq_engine_step(lq_engine* e, int32_t nsteps) {
             }
             LQ_HIP_TRY(hipGraphLaunch(g, e->stream));
         } else {
-            enqueue_step(e, e->cur, shep, nullptr);
+            (void)enqueue_step(e, e->cur, shep, nullptr);
             LQ_TRY(last_launch("lq_engine_step"));
         }
-        if (shep) e->cur = 1 - e->cur;
+        e->cur = step_out(e->cur, shep);
         host_step_end(e);
     }
     return LQ_OK;
@@ -869,7 +931,7 @@ extern "C" lq_status lq_engine_neighbour_counts(lq_engine* e, int32_t run_id, in
     const IndexMap mats{b.mat_idx, 0};
     const int var = e->variant >= 0 ? e->variant : g_variant;
     launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
-                             (int)e->nmat_total, (int)e->nmat_art, nbr, var, s);
+                             (int)e->nmat_total, (int)e->nmat_art, nbr, var, nullptr, s);
     const RunDesc& d = e->runs[i].desc;
     std::vector<int32_t> c(d.np), orig(d.np);
     hipError_t err = hipGetLastError();
@@ -914,11 +976,11 @@ extern "C" lq_status lq_engine_autotune(lq_engine* e, int32_t nrep, int32_t* bes
     int best_v = 3;
     for (int v = 0; v < LQ_INTERACTION_VARIANTS; ++v) {
         launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
-                                 (int)e->nmat_total, (int)e->nmat_art, nullptr, v, s);  // warm-up
+                                 (int)e->nmat_total, (int)e->nmat_art, nullptr, v, nullptr, s);  // warm-up
         LQ_HIP_TRY(hipEventRecord(ev[0], s));
         for (int r = 0; r < nrep; ++r)
             launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
-                                     (int)e->nmat_total, (int)e->nmat_art, nullptr, v, s);
+                                     (int)e->nmat_total, (int)e->nmat_art, nullptr, v, nullptr, s);
         LQ_HIP_TRY(hipEventRecord(ev[1], s));
         LQ_HIP_TRY(hipEventSynchronize(ev[1]));
         float ms = 0.f;
@@ -953,8 +1015,7 @@ extern "C" lq_status lq_engine_profile(lq_engine* e, int32_t nsteps, double* ms_
         for (const RunSlot& r : e->runs) any_active = any_active || r.desc.active;
         if (!any_active) break;
         const bool shep = any_shepard_due(e);
-        enqueue_step(e, e->cur, shep, &p);
-        if (shep) e->cur = 1 - e->cur;
+        e->cur = enqueue_step(e, e->cur, shep, &p);
         host_step_end(e);
         ++done;
     }

@@ -80,26 +80,35 @@ enum Stage { ST_ADAMI = 0, ST_INTERACT = 1, ST_PREDICT = 2, ST_CORRECT = 3, ST_S
 // particle index of the t-th wall / material particle.
 // variant >= 3 selects the list-based neighbour scan for Adami and Shepard
 // (same sums in the same order as the direct scan).
+// stale (nullable): the cell table was built before the particles last moved
+// (positions then and a displacement bound); the scans widen their reach by
+// the bound and test the current cells, so that the neighbour sets are those
+// of a fresh table (list-based kernels only; nbr: per-particle counts).
 void launch_adami_one(const RunDesc& d, const DevState& st, const int32_t* beg, const int32_t* end, int variant,
-                      hipStream_t s);
+                      const StaleIn* stale, int32_t* nbr, hipStream_t s);
 void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const int32_t* beg,
-                        const int32_t* end, IndexMap walls, int nwalls, int variant, hipStream_t s);
+                        const int32_t* end, IndexMap walls, int nwalls, int variant, const StaleIn* stale,
+                        hipStream_t s);
 void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates& rt, const int32_t* beg,
-                            const int32_t* end, int32_t* nbr, int variant, hipStream_t s);
+                            const int32_t* end, int32_t* nbr, int variant, const StaleIn* stale, hipStream_t s);
 // The last nart of the nmat material indices belong to runs with
 // artificial-pressure = 1; they get a kernel of their own so that the other
 // runs keep the exact arithmetic of the plain kernel.
 void launch_interaction_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt,
                               const int32_t* beg, const int32_t* end, IndexMap mats, int nmat, int nart,
-                              int32_t* nbr, int variant, hipStream_t s);
+                              int32_t* nbr, int variant, const StaleIn* stale, hipStream_t s);
 void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, const DevIni& ini,
                           const DevRates& rt, hipStream_t s);
 // ini_perm: the corrector reads particle k's ini entry at ini_perm[k] (the
 // last sort's permutation; nullable = identity).  fail: per-run flags raised
-// by the corrector on a non-finite state (nullable).
+// by the corrector on a non-finite state (nullable).  dmax (nullable, per
+// run, zeroed by launch_step_begin): the predictor records each run's largest
+// displacement (and nout, per run, whether a particle ended outside the
+// grid); the corrector raises fail when the displacement exceeded what a
+// stale table tolerates.
 void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
                             const DevIni& ini, const DevRates& rt, IndexMap mats, int nmat, const uint32_t* ini_perm,
-                            int32_t* fail, hipStream_t s);
+                            int32_t* fail, uint32_t* dmax, uint32_t* nout, hipStream_t s);
 // Shepard filter: reads stress from str_src, writes str_dst (walls and runs
 // not due are copied through).
 void launch_shepard_one(const RunDesc& d, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,
@@ -108,7 +117,9 @@ void launch_shepard_table(const RunDesc* d, const int32_t* run_of, const DevStat
                           tensor2d* str_dst, const int32_t* beg, const int32_t* end, int n, int variant,
                           hipStream_t s);
 // Per-run step bookkeeping on the device.
-void launch_step_begin(RunDesc* d, int nruns, hipStream_t s);
+void launch_step_begin(RunDesc* d, int nruns, uint32_t* dmax, uint32_t* nout, hipStream_t s);
+// Folds the predictor's per-run displacement slots (DMAX_SLOTS per run).
+void launch_dmax_fold(uint32_t* dmax, int nruns, hipStream_t s);
 void launch_step_end(RunDesc* d, int nruns, const int32_t* fail, hipStream_t s);
 // Dynamic time step of the runs with dynamic-time-step = 1: per-run maxima of
 // |v| (state) and |a| (rates of the previous corrector) into red (2 per run,

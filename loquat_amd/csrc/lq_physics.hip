@@ -548,6 +548,11 @@ __device__ __forceinline__ void pair_eval3(const IPart& pi, uint32_t j, const De
     pair_compute(pi, pair_load(st, j), c, a);
 }
 
+__device__ __forceinline__ float stale_disp(const StaleIn& sn, int run) {
+    return sn.dmax ? __uint_as_float(sn.dmax[run * DMAX_SLOTS]) : sn.disp;
+}
+__device__ __forceinline__ bool stale_out_now(const StaleIn& sn, int run) { return !sn.nout || sn.nout[run] != 0u; }
+
 // Wave-cooperative neighbour scan shared by the list-based kernels.  For a
 // lane with `active` set it walks the z-merged (x, y) columns of cell layers
 // [layer_lo, layer_hi] around cell (cix, ciy, ciz) -- (x, y) major, layers
@@ -572,12 +577,22 @@ __device__ __forceinline__ void pair_eval3(const IPart& pi, uint32_t j, const De
 // by 1e-3 and dxy_min is shrunk by 1e-3 of a cell, far beyond the FP32
 // rounding of the coordinates, so every skipped candidate has r2 > r2_cut.
 struct ScanGeom {
-    int cix, ciy, ciz, nx, ny, nz, nol, cbase;
+    int cix, ciy, ciz, nx, ny, nz, nol, cbase;  // stencil cell and grid
+    int czlo, czhi;  // z cells read in every column
     int zbits;
-    float u, v;     // position inside the cell along x and y, in cell units
+    float i_h;
+    float u, v;     // position inside the stencil cell along x and y, in cell units
     float cw;       // cell width (2 h)
     float rw2;      // (1.001 R)^2, R^2 = r2_cut
     float z, binw;  // particle z and the z-bin width widened by 1 %
+    // Stale tables: 0, or twice the largest displacement of any particle of
+    // the run since the table was built; whether the particle's current cell
+    // lies on (or beyond) the grid's edge, and whether some particle of the
+    // run lies outside the grid now.
+    float slack;
+    float ext2;  // (0.999 cell)^2
+    bool edge;
+    bool out_now;
 };
 
 __device__ __forceinline__ ScanGeom make_geom(const RunDesc& d, float4 ps) {
@@ -591,7 +606,10 @@ __device__ __forceinline__ ScanGeom make_geom(const RunDesc& d, float4 ps) {
     g.nz = d.par.cell_num_z;
     g.nol = d.par.ncell_one_layer;
     g.cbase = d.cell_base;
+    g.czlo = max(g.ciz - 1, 0);
+    g.czhi = min(g.ciz + 1, g.nz - 1);
     g.zbits = d.zbits;
+    g.i_h = i_h;
     const float sx = (0.5f * ps.x) * i_h, sy = (0.5f * ps.y) * i_h;
     g.u = sx - floorf(sx);
     g.v = sy - floorf(sy);
@@ -600,13 +618,45 @@ __device__ __forceinline__ ScanGeom make_geom(const RunDesc& d, float4 ps) {
     g.rw2 = rw * rw;
     g.z = ps.z;
     g.binw = 1.01f * g.cw / (float)(1 << d.zbits);
+    g.slack = 0.0f;
+    g.ext2 = (0.999f * g.cw) * (0.999f * g.cw);
+    g.edge = false;
+    g.out_now = false;
     return g;
+}
+
+// Geometry for a table built when the particle was at `po` (ps: now), every
+// particle of the run having moved at most `disp` since.  The (x, y) stencil
+// is taken around the old cell; the z cells are those a neighbour's old z can
+// lie in, |z_old - z| <= 1.001 R + disp (plus rounding).
+__device__ __forceinline__ ScanGeom make_geom_stale(const RunDesc& d, float4 ps, float4 po, float disp, bool out_now) {
+    ScanGeom g = make_geom(d, po);
+    const float i_h = g.i_h;
+    const int ccx = cell_coord(ps.x, i_h), ccy = cell_coord(ps.y, i_h), ccz = cell_coord(ps.z, i_h);
+    g.z = ps.z;
+    g.slack = 2.0f * disp;
+    const float reach = sqrtf(g.rw2) + disp + 1e-3f * g.cw;
+    g.czlo = max(cell_coord(ps.z - reach, i_h), 0);
+    g.czhi = min(cell_coord(ps.z + reach, i_h), g.nz - 1);
+    g.edge = ccx <= 0 || ccx >= g.nx - 1 || ccy <= 0 || ccy >= g.ny - 1 || ccz <= 0 || ccz >= g.nz - 1;
+    g.out_now = out_now;
+    return g;
+}
+
+// A candidate q is in the stencil of the current cell of the particle at
+// ps: q's current cell lies inside the grid and within one cell of it along
+// every axis -- the neighbour set of a freshly sorted table.
+__device__ __forceinline__ bool in_current_stencil(const ScanGeom& g, float4 ps, float4 q) {
+    const int x = cell_coord(q.x, g.i_h), y = cell_coord(q.y, g.i_h), z = cell_coord(q.z, g.i_h);
+    return x >= 0 && x < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz &&
+           abs(x - cell_coord(ps.x, g.i_h)) <= 1 && abs(y - cell_coord(ps.y, g.i_h)) <= 1 &&
+           abs(z - cell_coord(ps.z, g.i_h)) <= 1;
 }
 
 //
 // B = 2 evaluates the list two entries at a time, eval(j0, act0, j1, act1),
 // so that both gathers are in flight before either pair is computed.
-template <int B, bool PADDED, int CAP = IC2_CAP, class Eval>
+template <int B, bool PADDED, int CAP = IC2_CAP, bool STALE = false, class Eval>
 __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer_hi, const ScanGeom& g, float4 ps,
                                             float r2cut, const int32_t* __restrict__ beg,
                                             const int32_t* __restrict__ end, const float4* __restrict__ pp,
@@ -638,20 +688,55 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
         cnt = 0;
     };
     const float* __restrict__ pz = reinterpret_cast<const float*>(pp) + 2;  // pz[4 j] = z of particle j
-    const int czlo = max(g.ciz - 1, 0), czhi = min(g.ciz + 1, g.nz - 1);
+    const int czlo = g.czlo, czhi = g.czhi;
     const int nl = layer_hi - layer_lo + 1;
-    const int ncol = 9 * nl;
     const float margin = 1e-3f * g.cw;
+    // Stale tables: a neighbour's old position may lie one ring of columns
+    // further out (read only when some lane's particle is that close to its
+    // old cell's side), and one that was outside the grid when the table was
+    // built sits in the run's outside cell (read by lanes on the grid's edge).
+    int nxy = 9;
+    bool scan_out = false;
+    // a hit that lies outside the grid now is possible only for a lane on
+    // the grid's edge of a run with particles outside
+    const unsigned long long edge_lanes = STALE ? __ballot(active && g.edge && g.out_now) : 0ull;
+    int out_b = -1, out_e = -1;
+    if constexpr (STALE) {
+        const float fm = fminf(fminf(g.u, 1.0f - g.u), fminf(g.v, 1.0f - g.v));
+        if (__ballot(active && (fm + 1.0f) * g.cw < sqrtf(g.rw2) + margin + g.slack)) nxy = 25;
+        if (active && g.edge) {
+            const int co = g.cbase + 2 * g.nol;
+            out_b = beg[co];
+            out_e = end[co];
+        }
+        scan_out = __ballot(out_b != -1) != 0ull;
+    }
+    const int ncol = nxy * nl + (scan_out ? 1 : 0);
     // [b0, e0) of the column's cells czlo..czhi starting at cell id c0 (b0 =
     // -1 when all are empty).  The (up to) three cells' six table entries are
     // loaded together; particles of consecutive cells are consecutive, so the
     // range runs from the first non-empty cell's begin to the last one's end.
     auto col_range = [&](int c0, int& b0, int& e0) {
-        const int ca = c0 + czlo, cb = c0 + min(czlo + 1, czhi), cc = c0 + czhi;
-        const int ba = beg[ca], bb = beg[cb], bc = beg[cc];
-        const int ea = end[ca], eb = end[cb], ec = end[cc];
-        b0 = ba != -1 ? ba : (bb != -1 ? bb : bc);
-        e0 = bc != -1 ? ec : (bb != -1 ? eb : ea);
+        if constexpr (STALE) {
+            // up to four cells (the old z of a neighbour spans a little more
+            // than three cells)
+            const int ca = c0 + czlo, cb = c0 + min(czlo + 1, czhi), cc = c0 + min(czlo + 2, czhi), cd = c0 + czhi;
+            const int ba = beg[ca], bb = beg[cb], bc = beg[cc];
+            const int ea = end[ca], eb = end[cb], ec = end[cc];
+            int bd = -1, ed = -1;
+            if (czhi - czlo == 3) {  // rare: the fourth cell
+                bd = beg[cd];
+                ed = end[cd];
+            }
+            b0 = ba != -1 ? ba : (bb != -1 ? bb : (bc != -1 ? bc : bd));
+            e0 = bd != -1 ? ed : (bc != -1 ? ec : (bb != -1 ? eb : ea));
+        } else {
+            const int ca = c0 + czlo, cb = c0 + min(czlo + 1, czhi), cc = c0 + czhi;
+            const int ba = beg[ca], bb = beg[cb], bc = beg[cc];
+            const int ea = end[ca], eb = end[cb], ec = end[cc];
+            b0 = ba != -1 ? ba : (bb != -1 ? bb : bc);
+            e0 = bc != -1 ? ec : (bb != -1 ? eb : ea);
+        }
     };
     // First candidate of the non-empty range [b0, e0) that may lie in the
     // z-window.  A block of U whose last particle is below zlo lies wholly
@@ -680,10 +765,25 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
     float zhin = INFINITY;
     for (int col = 0; col < ncol; ++col) {
         const int xy = col / nl, layer = layer_lo + col % nl;
-        const int ox = xy / 3 - 1, oy = xy % 3 - 1;
+        int ox, oy;
+        if (xy < 9) {
+            ox = xy / 3 - 1;
+            oy = xy % 3 - 1;
+        } else {
+            // the ring of 16 columns two cells out
+            const int r = xy - 9;
+            ox = r < 5 ? -2 : (r < 10 ? 2 : (r < 13 ? r - 11 : r - 14));
+            oy = r < 5 ? r - 2 : (r < 10 ? r - 7 : (r < 13 ? -2 : 2));
+        }
         int j = 0, je = 0;
         float zhi = INFINITY;
-        if (layer != layer_lo) {
+        if (STALE && col == nxy * nl) {
+            // the run's outside cell, no z order
+            if (out_b != -1) {
+                j = out_b;
+                je = out_e;
+            }
+        } else if (layer != layer_lo) {
             j = jn;
             je = jen;
             zhi = zhin;
@@ -692,8 +792,15 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
             zhin = INFINITY;
             if (active && czlo <= czhi) {
                 const int cx = g.cix + ox, cy = g.ciy + oy;
-                const float ex = ox == 0 ? 0.0f : fmaxf((ox < 0 ? g.u : 1.0f - g.u) * g.cw - margin, 0.0f);
-                const float ey = oy == 0 ? 0.0f : fmaxf((oy < 0 ? g.v : 1.0f - g.v) * g.cw - margin, 0.0f);
+                // distance from the particle's (old) position to the column,
+                // less the rounding margin and the slack
+                const float sh = margin + g.slack;
+                const float ex =
+                    ox == 0 ? 0.0f
+                            : fmaxf(((ox < 0 ? g.u : 1.0f - g.u) + (float)(abs(ox) - 1)) * g.cw - sh, 0.0f);
+                const float ey =
+                    oy == 0 ? 0.0f
+                            : fmaxf(((oy < 0 ? g.v : 1.0f - g.v) + (float)(abs(oy) - 1)) * g.cw - sh, 0.0f);
                 const float hz2 = g.rw2 - (ex * ex + ey * ey);
                 if (hz2 > 0.0f && cx >= 0 && cx < g.nx && cy >= 0 && cy < g.ny) {
                     const int c0 = g.cbase + layer * g.nol + (cx * g.ny + cy) * g.nz;
@@ -702,9 +809,11 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
                     if (nl == 2) col_range(c0 + g.nol, b1, e1);
                     float zlo = -INFINITY;
                     if (g.zbits) {
+                        // the slack also covers the order of the z-bins,
+                        // which is that of the old positions
                         const float hz = sqrtf(hz2);
-                        zlo = g.z - hz - g.binw;
-                        zhi = g.z + hz + g.binw;
+                        zlo = g.z - hz - g.binw - g.slack;
+                        zhi = g.z + hz + g.binw + g.slack;
                     }
                     if (b1 >= 0) {
                         jn = g.zbits ? skip_below(b1, e1, zlo) : b1;
@@ -738,12 +847,27 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
                     for (int u = 0; u < U; ++u) q[u] = ldo(pp, min((uint32_t)(j + u), jlast));
                 }
                 int hit[U];
+                unsigned long long check = 0ull;
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     // both tests evaluated, combined without a branch
-                    const int close = r2_fp32(ps.x - q[u].x, ps.y - q[u].y, ps.z - q[u].z) < r2cut ? 1 : 0;
+                    const float r2 = r2_fp32(ps.x - q[u].x, ps.y - q[u].y, ps.z - q[u].z);
+                    const int close = r2 < r2cut ? 1 : 0;
                     const int inside = (j + u < je) ? 1 : 0;
                     hit[u] = close & inside;
+                    // Closer than 0.999 cell (so along every axis), two
+                    // particles' cells differ by at most one (FP32 rounding
+                    // of the cell coordinate is ~1e-5 cell for grids under
+                    // 8000 cells per axis), so only hits at that distance or
+                    // of a lane on the grid's edge need the exact cell test.
+                    if constexpr (STALE) check |= __ballot(r2 >= g.ext2 && r2 < r2cut);
+                }
+                if constexpr (STALE) {
+                    if (check | edge_lanes) {
+#pragma unroll
+                        for (int u = 0; u < U; ++u)
+                            if (hit[u] && !in_current_stencil(g, ps, q[u])) hit[u] = 0;
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -771,12 +895,12 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
 // zeroes the kernel gradient and v_i - v_i the velocity terms), so it changes
 // no sum, and one is taken off the neighbour count.  Variant 4 (B = 2)
 // evaluates the lists two pairs at a time; same pairs, same order.
-template <class DP, int B, bool MASKED = false, bool ART = false>
-__global__ __launch_bounds__(256) void k_interaction_v3(
-    DP dp, DevState st, DevRates rt,
+template <class DP, int B, bool MASKED = false, bool ART = false, bool STALE = false>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(STALE ? 4 : 1))) void
+k_interaction_v3(DP dp, DevState st, DevRates rt,
                                                         const int32_t* __restrict__ beg,
                                                         const int32_t* __restrict__ end, IndexMap mats, int n,
-                                                        int32_t* __restrict__ nbr) {
+                                                        int32_t* __restrict__ nbr, StaleIn sn = {}) {
     __shared__ int32_t lists[4][IC3_CAP][64];
     const int t = xcd_thread();
     const int k = t < n ? mats(t) : -1;
@@ -799,10 +923,17 @@ __global__ __launch_bounds__(256) void k_interaction_v3(
         r2cut = d.r2_cut;
         ps = st.pospres[k];
         load_part(st, k, pi);
-        g = make_geom(d, ps);
+        if constexpr (STALE) {
+            const float4 po = make_float4((float)sn.xo[k], (float)sn.yo[k], (float)sn.zo[k], 0.0f);
+            g = make_geom_stale(d, ps, po, stale_disp(sn, dp.run(k)), stale_out_now(sn, dp.run(k)));
+        } else {
+            g = make_geom(d, ps);
+        }
     }
-    const bool self_in =
-        mat && g.cix >= 0 && g.cix < g.nx && g.ciy >= 0 && g.ciy < g.ny && g.ciz >= 0 && g.ciz < g.nz;
+    // the particle is its own candidate exactly when its current cell lies
+    // inside the grid
+    const int ccx = cell_coord(ps.x, c.i_h), ccy = cell_coord(ps.y, c.i_h), ccz = cell_coord(ps.z, c.i_h);
+    const bool self_in = mat && ccx >= 0 && ccx < g.nx && ccy >= 0 && ccy < g.ny && ccz >= 0 && ccz < g.nz;
     IAcc a = {};
     // dummy = the particle itself: its pair terms are exactly zero
     int count;
@@ -821,11 +952,11 @@ __global__ __launch_bounds__(256) void k_interaction_v3(
         }
     };
     if constexpr (B == 1) {
-        count = scan_compact<1, DP::padded, IC3_CAP>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+        count = scan_compact<1, DP::padded, IC3_CAP, STALE>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
                                             mat ? (uint32_t)k : 0u,
                                             [&](uint32_t j, bool act) { pair_compute<ART>(pi, load(j, act), c, a); });
     } else {
-        count = scan_compact<2, DP::padded, IC3_CAP>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+        count = scan_compact<2, DP::padded, IC3_CAP, STALE>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
                                             mat ? (uint32_t)k : 0u,
                                             [&](uint32_t j0, bool a0, uint32_t j1, bool a1) {
                                                 const PairIn q0 = load(j0, a0);
@@ -842,9 +973,10 @@ __global__ __launch_bounds__(256) void k_interaction_v3(
 
 // Adami wall stress with the list-based scan (material layer only; a wall is
 // never in its own candidate stream).  Same sums as k_adami, same order.
-template <class DP>
+template <class DP, bool STALE = false>
 __global__ __launch_bounds__(256) void k_adami_lds(DP dp, DevState st, const int32_t* __restrict__ beg,
-                                                   const int32_t* __restrict__ end, IndexMap walls, int n) {
+                                                   const int32_t* __restrict__ end, IndexMap walls, int n,
+                                                   StaleIn sn = {}, int32_t* __restrict__ nbr = nullptr) {
     __shared__ int32_t lists[4][IC2_CAP][64];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;  // round-robin XCDs, as k_adami
     const int k = t < n ? walls(t) : -1;
@@ -870,11 +1002,14 @@ __global__ __launch_bounds__(256) void k_adami_lds(DP dp, DevState st, const int
         xi = st.xd[k];
         yi = st.yd[k];
         zi = st.zd[k];
-        g = make_geom(d, ps);
+        // walls do not move: the table's position is the current one
+        g = STALE ? make_geom_stale(d, ps, ps, stale_disp(sn, dp.run(k)), stale_out_now(sn, dp.run(k)))
+                  : make_geom(d, ps);
     }
     float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
     float px = 0.f, py = 0.f, pz = 0.f, wsum = 0.f;
-    scan_compact<1, DP::padded>(on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane, on ? (uint32_t)k : 0u,
+    const int cnt = scan_compact<1, DP::padded, IC2_CAP, STALE>(on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w],
+                                                                 lane, on ? (uint32_t)k : 0u,
                              [&](uint32_t j, bool act) {
         const float dx = (float)(xi - ldo(st.xd, j)), dy = (float)(yi - ldo(st.yd, j)),
                     dz = (float)(zi - ldo(st.zd, j));
@@ -895,6 +1030,7 @@ __global__ __launch_bounds__(256) void k_adami_lds(DP dp, DevState st, const int
         pz += dz * kv;
         wsum += kv;
     });
+    if (on && nbr) nbr[k] = cnt;
     if (on && wsum > 0.0f) {
         const RunDesc& d = *dptr;
         const float iw = 1.0f / wsum;
@@ -1066,16 +1202,26 @@ __device__ __forceinline__ void constitutive(const RunDesc& d, float dt, const f
     }
 }
 
+// Maximum of v over the wave's lanes with run >= 0, into red[stride * run +
+// off] (float bits; v >= 0): one atomic per wave when all contributing lanes
+// share a run, else one per lane.  Every lane of the wave must call it.
+__device__ __forceinline__ void wave_max_by_run(int run, float v, uint32_t* red, int stride, int off) {
+    const unsigned long long has = __ballot(run >= 0);
+    if (has == 0ull) return;  // uniform over the wave
+    const int r0 = __shfl(run, __ffsll((long long)has) - 1, 64);
+    if (__ballot(run >= 0 && run != r0) == 0ull) {
+        for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+        if ((threadIdx.x & 63) == 0) atomicMax(&red[stride * r0 + off], __float_as_uint(v));
+    } else if (run >= 0) {
+        atomicMax(&red[stride * run + off], __float_as_uint(v));
+    }
+}
+
 // Predictor (src/predictor_corrector.cu:169-218) fused with the *_ini copy
-// that the reference does as five device-to-device copies (:240-246).
-template <class DP>
-__global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni ini, DevRates rt, IndexMap mats,
-                                                   int n) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int k = mats(t);
-    const RunDesc& d = dp.get(dp.run(k));
-    if ((k - d.p_off) < d.npb || !d.active) return;
+// that the reference does as five device-to-device copies (:240-246), for
+// one material particle; returns how far it moved (rounded up).
+__device__ __forceinline__ float predict_one(const RunDesc& d, int k, const DevState& st, const DevIni& ini,
+                                             const DevRates& rt, bool& outside) {
     const float dt = d.par.dt;
     const float hdt = 0.5f * dt;
     const double hdtd = (double)hdt;
@@ -1117,6 +1263,89 @@ __global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni in
     st.idep[k] = idep;
     st.str[k] = s;
     st.stn[k] = stn;
+    const float i_h = d.par.i_h;
+    const int cx = cell_coord((float)x, i_h), cy = cell_coord((float)y, i_h), cz = cell_coord((float)z, i_h);
+    outside = cx < 0 || cx >= d.par.cell_num_x || cy < 0 || cy >= d.par.cell_num_y || cz < 0 ||
+              cz >= d.par.cell_num_z;
+    // |x_new - x_old| = hdt |v|, inflated far beyond the rounding of either
+    const float vn = sqrtf(vr.x * vr.x + vr.y * vr.y + vr.z * vr.z);
+    return hdt * vn * 1.0001f + 1e-12f;
+}
+
+// dmax (nullable, DMAX_SLOTS per run): the largest displacement of the run's
+// particles, for the corrector half's scans over this half step's (now
+// stale) table.  The workgroup's maximum goes to slot blockIdx % DMAX_SLOTS
+// with one atomic when all its particles belong to one run.  nout (with
+// dmax, per run): set when a particle of the run ends outside the grid.
+template <class DP>
+__global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni ini, DevRates rt, IndexMap mats,
+                                                   int n, uint32_t* __restrict__ dmax, uint32_t* __restrict__ nout) {
+    __shared__ float wmax[4];
+    __shared__ int wrun[4];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int run = -1;
+    float disp = 0.0f;
+    if (t < n) {
+        const int k = mats(t);
+        const int r = dp.run(k);
+        const RunDesc& d = dp.get(r);
+        if ((k - d.p_off) >= d.npb && d.active) {
+            run = r;
+            bool outside = false;
+            disp = predict_one(d, k, st, ini, rt, outside);
+            if (outside && nout) atomicOr(&nout[r], 1u);
+        }
+    }
+    if (!dmax) return;
+    uint32_t* slots = dmax + (blockIdx.x % DMAX_SLOTS);
+    // wave maximum when the wave's contributing lanes share a run (-1: none;
+    // -2: mixed, the lanes then add their own)
+    const unsigned long long has = __ballot(run >= 0);
+    int wr = -1;
+    if (has != 0ull) {
+        const int r0 = __shfl(run, __ffsll((long long)has) - 1, 64);
+        if (__ballot(run >= 0 && run != r0) == 0ull) {
+            for (int o = 32; o > 0; o >>= 1) disp = fmaxf(disp, __shfl_xor(disp, o, 64));
+            wr = r0;
+        } else {
+            wr = -2;
+            if (run >= 0) atomicMax(&slots[run * DMAX_SLOTS], __float_as_uint(disp));
+        }
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        wmax[w] = disp;
+        wrun[w] = wr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (int)(blockDim.x + 63) / 64;
+        int r0 = -1;
+        float m = 0.0f;
+        bool same = true;
+        for (int i = 0; i < nw; ++i) {
+            if (wrun[i] < 0) continue;
+            if (r0 < 0) r0 = wrun[i];
+            if (wrun[i] != r0) same = false;
+        }
+        if (same && r0 >= 0) {
+            for (int i = 0; i < nw; ++i)
+                if (wrun[i] == r0) m = fmaxf(m, wmax[i]);
+            atomicMax(&slots[r0 * DMAX_SLOTS], __float_as_uint(m));
+        } else {
+            for (int i = 0; i < nw; ++i)
+                if (wrun[i] >= 0) atomicMax(&slots[wrun[i] * DMAX_SLOTS], __float_as_uint(wmax[i]));
+        }
+    }
+}
+
+// Folds each run's displacement slots into slot 0.
+__global__ void k_dmax_fold(uint32_t* __restrict__ dmax, int nruns) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nruns) return;
+    uint32_t m = 0u;
+    for (int i = 0; i < DMAX_SLOTS; ++i) m = max(m, dmax[r * DMAX_SLOTS + i]);
+    dmax[r * DMAX_SLOTS] = m;
 }
 
 // Corrector (src/predictor_corrector.cu:256-307): midpoint update from ini.
@@ -1129,7 +1358,8 @@ __global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni in
 template <class DP>
 __global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni ini, DevRates rt, IndexMap mats,
                                                    int n, const uint32_t* __restrict__ ini_perm,
-                                                   int32_t* __restrict__ fail) {
+                                                   int32_t* __restrict__ fail,
+                                                   const uint32_t* __restrict__ dmax = nullptr) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int k = mats(t);
@@ -1159,6 +1389,10 @@ __global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni in
     if (fail) {
         const float chk = (vr.x + vr.y + vr.z + vr.w) + (s.xx + s.yy + s.zz) + (s.xy + s.xz + s.yz);
         if (!isfinite(chk) || !isfinite(x + y + z)) fail[run] = 1;
+        // the corrector half scanned a stale table: its neighbour sets are
+        // exact only within the displacement bound
+        if (dmax && !(2.0f * __uint_as_float(dmax[run * DMAX_SLOTS]) <= STALE_LIMIT * (2.0f / d.par.i_h)))
+            fail[run] = 1;
     }
 
     st.xd[k] = x;
@@ -1278,9 +1512,12 @@ __global__ __launch_bounds__(256) void k_shepard_lds(DP dp, DevState st, const t
     }
 }
 
-__global__ void k_step_begin(RunDesc* d, int nruns) {
+__global__ void k_step_begin(RunDesc* d, int nruns, uint32_t* __restrict__ dmax, uint32_t* __restrict__ nout) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nruns) return;
+    if (nout) nout[r] = 0u;
+    if (dmax)
+        for (int i = 0; i < DMAX_SLOTS; ++i) dmax[r * DMAX_SLOTS + i] = 0u;
     const int f = d[r].par.stress_shepard_filter;
     const int step = d[r].step;
     d[r].shepard_due = (d[r].active && f != 0 && step % f == 0 && step != 0) ? 1 : 0;
@@ -1328,24 +1565,8 @@ __global__ __launch_bounds__(256) void k_dt_reduce(const RunDesc* __restrict__ d
             r = -1;
         }
     }
-    // wave-wide maxima when every contributing lane has the same run
-    const unsigned long long has = __ballot(r >= 0);
-    if (has == 0ull) return;  // uniform over the wave
-    const int r0 = __shfl(r, __ffsll((long long)has) - 1, 64);
-    const bool same = __ballot(r >= 0 && r != r0) == 0ull;
-    if (same) {
-        for (int off = 32; off > 0; off >>= 1) {
-            vm = fmaxf(vm, __shfl_xor(vm, off, 64));
-            am = fmaxf(am, __shfl_xor(am, off, 64));
-        }
-        if ((threadIdx.x & 63) == 0) {
-            atomicMax(&red[2 * r0], __float_as_uint(vm));
-            atomicMax(&red[2 * r0 + 1], __float_as_uint(am));
-        }
-    } else if (r >= 0) {
-        atomicMax(&red[2 * r], __float_as_uint(vm));
-        atomicMax(&red[2 * r + 1], __float_as_uint(am));
-    }
+    wave_max_by_run(r, vm, red, 2, 0);
+    wave_max_by_run(r, am, red, 2, 1);
 }
 
 // Dynamic time step, part 2: dt = cfl * min(h / (cs + |v|max), sqrt(h / |a|max))
@@ -1377,29 +1598,36 @@ __global__ void k_copy_tensor(const tensor2d* __restrict__ src, tensor2d* __rest
 // Variants 3 and up use the list-based neighbour scan, the others the direct one.
 template <class DP>
 static void launch_adami(const DP& dp, const DevState& st, const int32_t* beg, const int32_t* end, IndexMap walls,
-                         int n, int variant, hipStream_t s) {
+                         int n, int variant, const StaleIn* stale, int32_t* nbr, hipStream_t s) {
     if (n <= 0) return;
-    if (variant >= 3)
-        k_adami_lds<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, beg, end, walls, n);
+    if (stale)
+        k_adami_lds<DP, true><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, beg, end, walls, n, *stale, nbr);
+    else if (variant >= 3)
+        k_adami_lds<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, beg, end, walls, n, StaleIn{}, nbr);
     else
         k_adami<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, beg, end, walls, n);
 }
 void launch_adami_one(const RunDesc& d, const DevState& st, const int32_t* beg, const int32_t* end, int variant,
-                      hipStream_t s) {
-    launch_adami(DescOne{d}, st, beg, end, IndexMap{nullptr, 0}, d.npb, variant, s);
+                      const StaleIn* stale, int32_t* nbr, hipStream_t s) {
+    launch_adami(DescOne{d}, st, beg, end, IndexMap{nullptr, 0}, d.npb, variant, stale, nbr, s);
 }
 void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const int32_t* beg,
-                        const int32_t* end, IndexMap walls, int nwalls, int variant, hipStream_t s) {
-    launch_adami(DescTable{d, run_of}, st, beg, end, walls, nwalls, variant, s);
+                        const int32_t* end, IndexMap walls, int nwalls, int variant, const StaleIn* stale,
+                        hipStream_t s) {
+    launch_adami(DescTable{d, run_of}, st, beg, end, walls, nwalls, variant, stale, nullptr, s);
 }
 
 template <class DP>
 static void launch_interaction(const DP& dp, const DevState& st, const DevRates& rt, const int32_t* beg,
                                const int32_t* end, IndexMap mats, int n, int32_t* nbr, int variant, bool art,
-                               hipStream_t s) {
+                               const StaleIn* stale, hipStream_t s) {
     if (n <= 0) return;
     const int nb = blocks_for(n, 256);
-    if (art)  // artificial pressure: one kernel (variant 6's shape) carries the extra term
+    if (stale && art)  // stale tables: variant 6's shape
+        k_interaction_v3<DP, 2, true, true, true><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
+    else if (stale)
+        k_interaction_v3<DP, 2, true, false, true><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
+    else if (art)  // artificial pressure: one kernel (variant 6's shape) carries the extra term
         k_interaction_v3<DP, 2, true, true><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else if (variant == 0)
         k_interaction_direct<DP><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
@@ -1418,17 +1646,18 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
 }
 
 void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates& rt, const int32_t* beg,
-                            const int32_t* end, int32_t* nbr, int variant, hipStream_t s) {
+                            const int32_t* end, int32_t* nbr, int variant, const StaleIn* stale, hipStream_t s) {
     launch_interaction(DescOne{d}, st, rt, beg, end, IndexMap{nullptr, d.npb}, d.np - d.npb, nbr, variant,
-                       d.par.artificial_pressure == 1, s);
+                       d.par.artificial_pressure == 1, stale, s);
 }
 void launch_interaction_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt,
                               const int32_t* beg, const int32_t* end, IndexMap mats, int nmat, int nart,
-                              int32_t* nbr, int variant, hipStream_t s) {
+                              int32_t* nbr, int variant, const StaleIn* stale, hipStream_t s) {
     const DescTable dt{d, run_of};
-    launch_interaction(dt, st, rt, beg, end, mats, nmat - nart, nbr, variant, false, s);
+    launch_interaction(dt, st, rt, beg, end, mats, nmat - nart, nbr, variant, false, stale, s);
     if (nart > 0)
-        launch_interaction(dt, st, rt, beg, end, IndexMap{mats.idx + (nmat - nart), 0}, nart, nbr, variant, true, s);
+        launch_interaction(dt, st, rt, beg, end, IndexMap{mats.idx + (nmat - nart), 0}, nart, nbr, variant, true,
+                           stale, s);
 }
 
 void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, const DevIni& ini,
@@ -1439,17 +1668,21 @@ void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, 
     if (corrector)
         k_corrector<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, mats, n, nullptr, nullptr);
     else
-        k_predictor<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, mats, n);
+        k_predictor<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, mats, n, nullptr, nullptr);
 }
 void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
                             const DevIni& ini, const DevRates& rt, IndexMap mats, int nmat, const uint32_t* ini_perm,
-                            int32_t* fail, hipStream_t s) {
+                            int32_t* fail, uint32_t* dmax, uint32_t* nout, hipStream_t s) {
     if (nmat <= 0) return;
     const int nb = blocks_for(nmat, 256);
     if (corrector)
-        k_corrector<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat, ini_perm, fail);
+        k_corrector<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat, ini_perm, fail,
+                                                  dmax);
     else
-        k_predictor<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat);
+        k_predictor<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat, dmax, nout);
+}
+void launch_dmax_fold(uint32_t* dmax, int nruns, hipStream_t s) {
+    if (nruns > 0) k_dmax_fold<<<blocks_for(nruns, 64), 64, 0, s>>>(dmax, nruns);
 }
 
 template <class DP>
@@ -1471,8 +1704,8 @@ void launch_shepard_table(const RunDesc* d, const int32_t* run_of, const DevStat
     launch_shepard(DescTable{d, run_of}, st, str_src, str_dst, beg, end, n, variant, s);
 }
 
-void launch_step_begin(RunDesc* d, int nruns, hipStream_t s) {
-    if (nruns > 0) k_step_begin<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns);
+void launch_step_begin(RunDesc* d, int nruns, uint32_t* dmax, uint32_t* nout, hipStream_t s) {
+    if (nruns > 0) k_step_begin<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns, dmax, nout);
 }
 void launch_dynamic_dt(RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt, IndexMap mats,
                        int nmat, int nruns, uint32_t* red, hipStream_t s) {

@@ -178,6 +178,22 @@ def interaction(par, st: DeviceState, rt: DeviceRates, beg: int, end: int, nbr: 
                                                 stream_ptr()))
 
 
+def interaction_stale(par, st: DeviceState, rt: DeviceRates, then: tuple, disp: float, beg: int, end: int,
+                      nbr: torch.Tensor | None = None) -> None:
+    """lq_particle_interaction_stale: then = (x, y, z) FP64 device tensors of
+    the positions the table was built from."""
+    s, r = st.struct(), rt.struct()
+    xo, yo, zo = (abi.ptr(t) for t in then)
+    abi.check(abi.lib().lq_particle_interaction_stale(C.byref(par), C.byref(s), C.byref(r), xo, yo, zo,
+                                                      C.c_float(disp), beg, end, abi.ptr(nbr), stream_ptr()))
+
+
+def adami_stale(par, st: DeviceState, disp: float, beg: int, end: int, nbr: torch.Tensor | None = None) -> None:
+    s = st.struct()
+    abi.check(abi.lib().lq_adami_boundary_stale(C.byref(par), C.byref(s), C.c_float(disp), beg, end,
+                                                abi.ptr(nbr), stream_ptr()))
+
+
 def predictor(par, vw, st: DeviceState, ini: DeviceIni, rt: DeviceRates) -> None:
     s, i, r = st.struct(), ini.struct(), rt.struct()
     abi.check(abi.lib().lq_predictor(C.byref(par), C.byref(vw), C.byref(s), C.byref(i), C.byref(r), stream_ptr()))

@@ -281,3 +281,122 @@ def test_shepard_parity(gpu):
         assert rel_err(h["str"][m, c], ost.s[c, m]) < TOL, c
     assert np.array_equal(h["str"][: p.npb], walls_before)
     g.close()
+
+
+def _brute_counts(p, pos32: np.ndarray, npb: int) -> np.ndarray:
+    """Neighbour counts of a fresh grid, by brute force on the FP32
+    positions: j != i whose cell (inside the grid) lies in the 27-cell stencil
+    of i's cell and r2 < r2_cut; walls count material neighbours only."""
+    from scipy.spatial import cKDTree
+    ih, r2c = np.float32(p.i_h), np.float32(abi.r2_cut(p))
+    cells = np.floor((np.float32(0.5) * pos32) * ih).astype(np.int64)
+    nmax = np.array([p.cell_num_x, p.cell_num_y, p.cell_num_z])
+    inside = np.all((cells >= 0) & (cells < nmax), axis=1)
+    pairs = cKDTree(pos32.astype(np.float64)).query_pairs(r=1.01 * float(np.sqrt(r2c)), output_type="ndarray")
+    i, j = pairs[:, 0], pairs[:, 1]
+    d = pos32[i] - pos32[j]
+    r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    near = np.all(np.abs(cells[i] - cells[j]) <= 1, axis=1) & (r2 < r2c)
+    mat = np.arange(len(pos32)) >= npb
+    either = mat[i] | mat[j]
+    cnt = np.zeros(len(pos32), np.int64)
+    np.add.at(cnt, i[near & inside[j] & either], 1)
+    np.add.at(cnt, j[near & inside[i] & either], 1)
+    return cnt
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("frac", [0.003, 0.1, 0.44])
+def test_stale_table_neighbour_sets_are_fresh(gpu, frac):
+    """Interaction and Adami on a table built before the particles moved (by
+    at most disp = frac * 2h, some exactly disp across cell faces, some out
+    of and back into the grid): neighbour counts equal a fresh grid's (brute
+    force), rates and wall stresses agree with the fresh-grid calls."""
+    dev = _dev()
+    import torch
+    c = cases.perturb(cases.small_column_3d(seed=31), seed=32)
+    p0 = c.derived()
+    cw = 2.0 / float(p0.i_h)
+    disp = frac * cw
+    npb = p0.npb
+    top = p0.cell_num_z * cw
+    # two rows of material particles at the grid's top face: one inside that
+    # leaves the grid, one outside that enters it
+    sel = np.argsort(c.posd[npb:, 2])[-6:] + npb
+    x0, y0 = c.posd[sel[0], 0], c.posd[sel[0], 1]
+    for r, k in enumerate(sel):
+        c.posd[k] = (x0 + 0.004 * (r % 3), y0, top + (0.3 if r >= 3 else -0.3) * disp)
+    c.pospres[sel, :3] = c.posd[sel].astype(np.float32)
+    p, dsA, ost = sorted_inputs(c)
+    g, beg, end, ref = _grid_tables(p, dsA, ost)
+    hA = dsA.host()
+    n = dsA.n
+    orig = hA["orig"]
+    rng = np.random.default_rng(33)
+    move = np.zeros((n, 3))
+    m = np.arange(npb, n)
+    u = rng.normal(size=(len(m), 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    move[m] = u * (rng.uniform(0.0, 1.0, len(m)) * disp)[:, None]
+    # 300 particles moved exactly disp towards their nearest x face
+    fx = (np.float32(0.5) * hA["posd"][:, 0].astype(np.float32)) * np.float32(p.i_h)
+    frx = fx - np.floor(fx)
+    pick = rng.choice(m, 300, replace=False)
+    move[pick] = 0.0
+    move[pick, 0] = np.where(frx[pick] < 0.5, -disp, disp)
+    k_sel = np.nonzero(np.isin(orig, sel))[0]
+    move[k_sel] = 0.0
+    move[k_sel, 2] = np.where(hA["posd"][k_sel, 2] < top, disp, -disp)
+    posB = hA["posd"] + move
+    assert np.linalg.norm(move, axis=1).max() <= disp * (1 + 1e-12)
+
+    def state_b():
+        return dev.DeviceState.from_host(posB, hA["pospres"], hA["velrhop"], hA["idep"], hA["str"], hA["stn"],
+                                         orig)
+
+    dsB = state_b()
+    then = (dsA.xd, dsA.yd, dsA.zd)
+    rt_s = dev.DeviceRates(n)
+    nbr_s = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+    dev.interaction_stale(p, dsB, rt_s, then, disp, beg, end, nbr_s)
+    dsBw = state_b()
+    nbr_ws = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+    dev.adami_stale(p, dsBw, disp, beg, end, nbr_ws)
+    # fresh grid of the moved particles
+    g2 = dev.Grid(n, p.ncell)
+    g2.build(p, dsB)
+    dsBs = dev.DeviceState(n)
+    g2.permute(n, dsB, dsBs)
+    beg2, end2 = g2.table_ptrs()
+    rt_f = dev.DeviceRates(n)
+    nbr_f = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+    dev.interaction(p, dsBs, rt_f, beg2, end2, nbr_f)
+    hBs = dsBs.host()
+    dev.adami(p, dsBs, beg2, end2)
+    hBsw = dsBs.host()
+    # fresh order -> stale order, by original index
+    pos_of = np.empty(n, np.int64)
+    pos_of[orig] = np.arange(n)
+    to_b = pos_of[hBs["orig"]]
+    bf = _brute_counts(p, posB.astype(np.float32), npb)
+    ns, nws = nbr_s.cpu().numpy(), nbr_ws.cpu().numpy()
+    nf = np.empty(n, np.int64)
+    nf[to_b] = nbr_f.cpu().numpy()
+    mat = slice(npb, None)
+    assert np.array_equal(nf[mat], bf[mat]), "fresh grid"
+    assert np.array_equal(ns[mat], bf[mat]), np.nonzero(ns[mat] != bf[mat])[0][:10]
+    assert np.array_equal(nws[:npb], bf[:npb]), np.nonzero(nws[:npb] != bf[:npb])[0][:10]
+    assert bf[k_sel].min() > 0  # the rows at the top face have neighbours
+    hs, hf = rt_s.host(), rt_f.host()
+    for key, w in (("accad", 4), ("spin", 3), ("acestn", 6)):
+        f = np.empty_like(hf[key])
+        f[to_b] = hf[key]
+        for col in range(w):
+            assert rel_err(hs[key][mat, col], f[mat, col]) < TOL, (key, col)
+    sw = dsBw.host()["str"]
+    fw = np.empty_like(hBsw["str"])
+    fw[to_b] = hBsw["str"]
+    for col in range(6):
+        assert rel_err(sw[:npb, col], fw[:npb, col]) < TOL, col
+    g.close()
+    g2.close()
