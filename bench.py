@@ -33,8 +33,17 @@ BYTES_WRITE_PER_MATERIAL = 52  # a (12) + drho (4) + spin (12) + strain rate (24
 BYTES_PER_CELL = 8             # cell_beg + cell_end
 WORKLOAD_ID = "column_3d dx=5mm npm=1024000"  # the key of this workload in profiles/*.json
 KERNEL_NAMES = {0: "k_interaction_direct", 1: "k_interaction_lds", 2: "k_interaction_unrolled",
-                3: "k_interaction_v3<DescTable, 1, false, false>", 4: "k_interaction_v3<DescTable, 2, false, false>",
-                5: "k_interaction_v3<DescTable, 1, true, false>", 6: "k_interaction_v3<DescTable, 2, true, false>"}
+                3: "k_interaction_v3<DescTable, 1, false, false, false>",
+                4: "k_interaction_v3<DescTable, 2, false, false, false>",
+                5: "k_interaction_v3<DescTable, 1, true, false, false>",
+                6: "k_interaction_v3<DescTable, 2, true, false, false>"}
+# The corrector half scans the predictor half's cell table with this kernel
+# (variant 6's shape), whatever variant the predictor half uses.
+STALE_KERNEL = "k_interaction_v3<DescTable, 2, true, false, true>"
+
+
+def roofline_kernel(variant: int) -> str:
+    return f"{KERNEL_NAMES[variant]} (predictor half) + {STALE_KERNEL} (corrector half)"
 
 
 def log(msg: str) -> None:
@@ -377,7 +386,7 @@ def main() -> None:
     bytes_launch = case.np * BYTES_READ_PER_PARTICLE + case.npm * BYTES_WRITE_PER_MATERIAL + p.ncell * BYTES_PER_CELL
     flops_launch = pairs * FLOPS_PER_PAIR
     gbs = bytes_launch / (inter_launch_ms * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic(KERNEL_NAMES[variant])
+    traffic, traffic_src = measured_traffic(roofline_kernel(variant))
     tflops = flops_launch / (inter_launch_ms * 1e-3) / 1e12
 
     value = case.npm * a.steps / (gpu_ms * 1e-3) / 1e6
@@ -406,8 +415,9 @@ def main() -> None:
             "steps_done": info["steps"],
         },
         "roofline": {
-            "kernel": KERNEL_NAMES[variant],
-            "measured_by": "HIP events around the interaction launches on the engine stream (lq_engine_profile)",
+            "kernel": roofline_kernel(variant),
+            "measured_by": "HIP events around the interaction launches on the engine stream (lq_engine_profile); "
+                           "launch_ms is the mean of the step's two launches",
             "bound": "hbm",
             "achieved": round(gbs, 2),
             "peak": HBM_PEAK_GBS,
