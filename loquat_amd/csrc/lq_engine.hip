@@ -366,7 +366,7 @@ struct Buffers {
     int32_t* run_of;
     RunDesc* desc;
     int32_t* fail;  // per-run flag, raised on the device by the corrector
-    uint32_t* dtred;  // per-run |v|max, |a|max of the dynamic time step (2 per run, kept zero between steps)
+    uint32_t* dtred;  // per-run |v|max, |a|max of the dynamic time step (2 x DMAX_SLOTS per run, zero between steps)
     uint32_t* dmax;   // per-run largest displacement of the predictor half (float bits, DMAX_SLOTS per run)
     uint32_t* nout;   // per run: some particle ended the predictor half outside the grid
     int32_t* mat_idx;   // particle index of the t-th material particle (all runs)
@@ -680,9 +680,9 @@ extern "C" lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t
          dmalloc(b.R.acestn, np_cap, e->owned) && dmalloc(b.T, np_cap, e->owned) &&
          dmalloc(b.run_of, np_cap, e->owned) && dmalloc(b.desc, max_runs, e->owned) &&
          dmalloc(b.fail, max_runs, e->owned) && dmalloc(b.mat_idx, np_cap, e->owned) &&
-         dmalloc(b.wall_idx, np_cap, e->owned) && dmalloc(b.dtred, 2 * (int64_t)max_runs, e->owned) &&
+         dmalloc(b.wall_idx, np_cap, e->owned) && dmalloc(b.dtred, 2 * (int64_t)max_runs * DMAX_SLOTS, e->owned) &&
          dmalloc(b.dmax, (int64_t)max_runs * DMAX_SLOTS, e->owned) && dmalloc(b.nout, max_runs, e->owned);
-    ok = ok && hipMemset(b.dtred, 0, 2 * (size_t)max_runs * sizeof(uint32_t)) == hipSuccess;
+    ok = ok && hipMemset(b.dtred, 0, 2 * (size_t)max_runs * DMAX_SLOTS * sizeof(uint32_t)) == hipSuccess;
     if (!ok || alloc_grid(b.grid, np_cap, ncell_cap) != LQ_OK) {
         for (void* p : e->owned) (void)hipFree(p);
         (void)hipStreamDestroy(e->stream);

@@ -122,8 +122,9 @@ void launch_step_begin(RunDesc* d, int nruns, uint32_t* dmax, uint32_t* nout, hi
 void launch_dmax_fold(uint32_t* dmax, int nruns, hipStream_t s);
 void launch_step_end(RunDesc* d, int nruns, const int32_t* fail, hipStream_t s);
 // Dynamic time step of the runs with dynamic-time-step = 1: per-run maxima of
-// |v| (state) and |a| (rates of the previous corrector) into red (2 per run,
-// zero on entry, cleared again on exit), then each such run's par.dt.
+// |v| (state) and |a| (rates of the previous corrector) into red (2 x
+// DMAX_SLOTS per run, zero on entry, cleared again on exit), then each such
+// run's par.dt.
 void launch_dynamic_dt(RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt, IndexMap mats,
                        int nmat, int nruns, uint32_t* red, hipStream_t s);
 // Copies material stress back (reference's D2D copy, src/field_smoothing.cu:144).

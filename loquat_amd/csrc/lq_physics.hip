@@ -1202,18 +1202,56 @@ __device__ __forceinline__ void constitutive(const RunDesc& d, float dt, const f
     }
 }
 
-// Maximum of v over the wave's lanes with run >= 0, into red[stride * run +
-// off] (float bits; v >= 0): one atomic per wave when all contributing lanes
-// share a run, else one per lane.  Every lane of the wave must call it.
-__device__ __forceinline__ void wave_max_by_run(int run, float v, uint32_t* red, int stride, int off) {
+// Maximum of v over the workgroup's threads with run >= 0, into the run's
+// slot blockIdx % DMAX_SLOTS of field f: red[(run * DMAX_SLOTS + slot) * nf +
+// f] (float bits, v >= 0).  One atomic per workgroup when all its
+// contributing threads share a run, so that a large run does not serialise
+// one atomic per wave on one address.  Every thread of the workgroup (at most
+// 256) must call it.
+__device__ __forceinline__ void block_max_by_run(int run, float v, uint32_t* red, int nf, int f) {
+    __shared__ float wmax[4];
+    __shared__ int wrun[4];
+    const int slot = (int)(blockIdx.x % DMAX_SLOTS);
+    auto at = [&](int r) { return &red[((size_t)r * DMAX_SLOTS + slot) * nf + f]; };
+    // wave maximum when the wave's contributing lanes share a run (-1: none;
+    // -2: mixed, the lanes then add their own)
     const unsigned long long has = __ballot(run >= 0);
-    if (has == 0ull) return;  // uniform over the wave
-    const int r0 = __shfl(run, __ffsll((long long)has) - 1, 64);
-    if (__ballot(run >= 0 && run != r0) == 0ull) {
-        for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-        if ((threadIdx.x & 63) == 0) atomicMax(&red[stride * r0 + off], __float_as_uint(v));
-    } else if (run >= 0) {
-        atomicMax(&red[stride * run + off], __float_as_uint(v));
+    int wr = -1;
+    if (has != 0ull) {
+        const int r0 = __shfl(run, __ffsll((long long)has) - 1, 64);
+        if (__ballot(run >= 0 && run != r0) == 0ull) {
+            for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+            wr = r0;
+        } else {
+            wr = -2;
+            if (run >= 0) atomicMax(at(run), __float_as_uint(v));
+        }
+    }
+    __syncthreads();  // the arrays may hold a previous call's values
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        wmax[w] = v;
+        wrun[w] = wr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (int)(blockDim.x + 63) / 64;
+        int r0 = -1;
+        bool same = true;
+        for (int i = 0; i < nw; ++i) {
+            if (wrun[i] < 0) continue;
+            if (r0 < 0) r0 = wrun[i];
+            if (wrun[i] != r0) same = false;
+        }
+        if (same && r0 >= 0) {
+            float m = 0.0f;
+            for (int i = 0; i < nw; ++i)
+                if (wrun[i] == r0) m = fmaxf(m, wmax[i]);
+            atomicMax(at(r0), __float_as_uint(m));
+        } else {
+            for (int i = 0; i < nw; ++i)
+                if (wrun[i] >= 0) atomicMax(at(wrun[i]), __float_as_uint(wmax[i]));
+        }
     }
 }
 
@@ -1274,14 +1312,11 @@ __device__ __forceinline__ float predict_one(const RunDesc& d, int k, const DevS
 
 // dmax (nullable, DMAX_SLOTS per run): the largest displacement of the run's
 // particles, for the corrector half's scans over this half step's (now
-// stale) table.  The workgroup's maximum goes to slot blockIdx % DMAX_SLOTS
-// with one atomic when all its particles belong to one run.  nout (with
-// dmax, per run): set when a particle of the run ends outside the grid.
+// stale) table, folded into slot 0 by k_dmax_fold.  nout (with dmax, per
+// run): set when a particle of the run ends outside the grid.
 template <class DP>
 __global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni ini, DevRates rt, IndexMap mats,
                                                    int n, uint32_t* __restrict__ dmax, uint32_t* __restrict__ nout) {
-    __shared__ float wmax[4];
-    __shared__ int wrun[4];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int run = -1;
     float disp = 0.0f;
@@ -1296,47 +1331,7 @@ __global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni in
             if (outside && nout) atomicOr(&nout[r], 1u);
         }
     }
-    if (!dmax) return;
-    uint32_t* slots = dmax + (blockIdx.x % DMAX_SLOTS);
-    // wave maximum when the wave's contributing lanes share a run (-1: none;
-    // -2: mixed, the lanes then add their own)
-    const unsigned long long has = __ballot(run >= 0);
-    int wr = -1;
-    if (has != 0ull) {
-        const int r0 = __shfl(run, __ffsll((long long)has) - 1, 64);
-        if (__ballot(run >= 0 && run != r0) == 0ull) {
-            for (int o = 32; o > 0; o >>= 1) disp = fmaxf(disp, __shfl_xor(disp, o, 64));
-            wr = r0;
-        } else {
-            wr = -2;
-            if (run >= 0) atomicMax(&slots[run * DMAX_SLOTS], __float_as_uint(disp));
-        }
-    }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        wmax[w] = disp;
-        wrun[w] = wr;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int nw = (int)(blockDim.x + 63) / 64;
-        int r0 = -1;
-        float m = 0.0f;
-        bool same = true;
-        for (int i = 0; i < nw; ++i) {
-            if (wrun[i] < 0) continue;
-            if (r0 < 0) r0 = wrun[i];
-            if (wrun[i] != r0) same = false;
-        }
-        if (same && r0 >= 0) {
-            for (int i = 0; i < nw; ++i)
-                if (wrun[i] == r0) m = fmaxf(m, wmax[i]);
-            atomicMax(&slots[r0 * DMAX_SLOTS], __float_as_uint(m));
-        } else {
-            for (int i = 0; i < nw; ++i)
-                if (wrun[i] >= 0) atomicMax(&slots[wrun[i] * DMAX_SLOTS], __float_as_uint(wmax[i]));
-        }
-    }
+    if (dmax) block_max_by_run(run, disp, dmax, 1, 0);
 }
 
 // Folds each run's displacement slots into slot 0.
@@ -1565,8 +1560,8 @@ __global__ __launch_bounds__(256) void k_dt_reduce(const RunDesc* __restrict__ d
             r = -1;
         }
     }
-    wave_max_by_run(r, vm, red, 2, 0);
-    wave_max_by_run(r, am, red, 2, 1);
+    block_max_by_run(r, vm, red, 2, 0);
+    block_max_by_run(r, am, red, 2, 1);
 }
 
 // Dynamic time step, part 2: dt = cfl * min(h / (cs + |v|max), sqrt(h / |a|max))
@@ -1576,15 +1571,21 @@ __global__ void k_dt_update(RunDesc* d, int nruns, uint32_t* __restrict__ red) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nruns) return;
     RunDesc& dr = d[r];
+    uint32_t vb = 0u, ab = 0u;
+    for (int i = 0; i < DMAX_SLOTS; ++i) {
+        uint32_t* sl = &red[((size_t)r * DMAX_SLOTS + i) * 2];
+        vb = max(vb, sl[0]);
+        ab = max(ab, sl[1]);
+        sl[0] = 0u;
+        sl[1] = 0u;
+    }
     if (dr.active && dr.par.dynamic_time_step == 1) {
-        const float vmax = __uint_as_float(red[2 * r]), amax = __uint_as_float(red[2 * r + 1]);
+        const float vmax = __uint_as_float(vb), amax = __uint_as_float(ab);
         const float h = dr.par.h;
         float dt = h / (dr.par.cs + vmax);
         if (amax > 0.0f) dt = fminf(dt, sqrtf(h / amax));
         dr.par.dt = dr.par.cfl * dt;
     }
-    red[2 * r] = 0u;
-    red[2 * r + 1] = 0u;
 }
 
 __global__ void k_copy_tensor(const tensor2d* __restrict__ src, tensor2d* __restrict__ dst, int n) {
