@@ -299,7 +299,10 @@ lq_status lq_engine_add_run(lq_engine* eng, const lq_run_input* in, int32_t* run
 lq_status lq_engine_remove_run(lq_engine* eng, int32_t run_id);
 
 /* Advances every unfinished run by up to nsteps steps (runs stop at their own
- * nsteps).  Asynchronous on the engine stream. */
+ * nsteps).  Asynchronous on the engine stream.  Each step searches neighbours
+ * once, in the predictor half; the corrector half reuses that cell table
+ * with the reach widened by how far the run's particles moved, which gives
+ * the neighbour sets of a second search (src/loquat.cu:362-366). */
 lq_status lq_engine_step(lq_engine* eng, int32_t nsteps);
 lq_status lq_engine_synchronize(lq_engine* eng);
 
@@ -326,8 +329,11 @@ lq_status lq_engine_download(lq_engine* eng, int32_t run_id, lq_host_particles* 
 lq_status lq_engine_compact(lq_engine* eng, int64_t* removed);
 
 /* Collects the per-run failure flags the corrector raises on the device when
- * a particle's new state is not finite (NaN/Inf).  A failed run stops
- * stepping at once; the other runs are unaffected.  Synchronous; nfailed
+ * a particle's new state is not finite (NaN/Inf), or when the run's
+ * particles moved more than 0.45 cell (0.9 h) in half a step, beyond what
+ * the corrector half's reuse of the cell table covers (no run within a CFL
+ * bound comes near it).  A failed run stops stepping at once; the other runs
+ * are unaffected.  Synchronous; nfailed
  * (nullable) receives the number of failed resident runs. */
 lq_status lq_engine_poll(lq_engine* eng, int32_t* nfailed);
 lq_status lq_engine_run_failed(const lq_engine* eng, int32_t run_id, int32_t* failed);
