@@ -894,13 +894,13 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
 // lies inside the grid): all its pair terms are exactly zero (x_i - x_i = 0
 // zeroes the kernel gradient and v_i - v_i the velocity terms), so it changes
 // no sum, and one is taken off the neighbour count.  Variant 4 (B = 2)
-// evaluates the lists two pairs at a time; same pairs, same order.
+// evaluates the lists two pairs at a time; same pairs, same order.  ART adds
+// the artificial pressure; STALE scans a table built before the particles
+// last moved (sn), held to four waves per SIMD (at most 128 VGPRs).
 template <class DP, int B, bool MASKED = false, bool ART = false, bool STALE = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(STALE ? 4 : 1))) void
-k_interaction_v3(DP dp, DevState st, DevRates rt,
-                                                        const int32_t* __restrict__ beg,
-                                                        const int32_t* __restrict__ end, IndexMap mats, int n,
-                                                        int32_t* __restrict__ nbr, StaleIn sn = {}) {
+k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ beg, const int32_t* __restrict__ end,
+                 IndexMap mats, int n, int32_t* __restrict__ nbr, StaleIn sn = {}) {
     __shared__ int32_t lists[4][IC3_CAP][64];
     const int t = xcd_thread();
     const int k = t < n ? mats(t) : -1;
