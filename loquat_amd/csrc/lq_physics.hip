@@ -573,9 +573,10 @@ __device__ __forceinline__ bool stale_out_now(const StaleIn& sn, int run) { retu
 // at the first block whose last particle lies above it.  The kernel is bound
 // by the latency of dependent load rounds, so each column's set-up issues its
 // loads together: both layers' cell-table entries in one round, the z-window
-// start in (usually) one more.  R is the support radius inflated
-// by 1e-3 and dxy_min is shrunk by 1e-3 of a cell, far beyond the FP32
-// rounding of the coordinates, so every skipped candidate has r2 > r2_cut.
+// start in (usually) one more.  R is the support radius inflated by eps and
+// dxy_min is shrunk by eps of a cell (eps = 1e-3, more on grids over ~2000
+// cells per axis), far beyond the FP32 rounding of the coordinates, so every
+// skipped candidate has r2 > r2_cut.
 struct ScanGeom {
     int cix, ciy, ciz, nx, ny, nz, nol, cbase;  // stencil cell and grid
     int czlo, czhi;  // z cells read in every column
@@ -583,14 +584,15 @@ struct ScanGeom {
     float i_h;
     float u, v;     // position inside the stencil cell along x and y, in cell units
     float cw;       // cell width (2 h)
-    float rw2;      // (1.001 R)^2, R^2 = r2_cut
+    float rw2;      // ((1 + eps) R)^2, R^2 = r2_cut
+    float eps;      // rounding allowance, in cells (and relative to R)
     float z, binw;  // particle z and the z-bin width widened by 1 %
     // Stale tables: 0, or twice the largest displacement of any particle of
     // the run since the table was built; whether the particle's current cell
     // lies on (or beyond) the grid's edge, and whether some particle of the
     // run lies outside the grid now.
     float slack;
-    float ext2;  // (0.999 cell)^2
+    float ext2;  // ((1 - eps) cell)^2
     bool edge;
     bool out_now;
 };
@@ -614,12 +616,17 @@ __device__ __forceinline__ ScanGeom make_geom(const RunDesc& d, float4 ps) {
     g.u = sx - floorf(sx);
     g.v = sy - floorf(sy);
     g.cw = 2.0f / i_h;
-    const float rw = 1.001f * sqrtf(d.r2_cut);
+    // FP32 rounding of a cell coordinate (at most a few ulps of the largest
+    // one) and of the squared distance stay far below eps = 1e-3 for grids of
+    // up to ~2000 cells per axis; larger grids get a proportionally larger eps
+    const int nmax = max(max(g.nx, g.ny), g.nz);
+    g.eps = fmaxf(1e-3f, 4.8e-7f * (float)nmax);
+    const float rw = (1.0f + g.eps) * sqrtf(d.r2_cut);
     g.rw2 = rw * rw;
     g.z = ps.z;
     g.binw = 1.01f * g.cw / (float)(1 << d.zbits);
     g.slack = 0.0f;
-    g.ext2 = (0.999f * g.cw) * (0.999f * g.cw);
+    g.ext2 = ((1.0f - g.eps) * g.cw) * ((1.0f - g.eps) * g.cw);
     g.edge = false;
     g.out_now = false;
     return g;
@@ -635,7 +642,7 @@ __device__ __forceinline__ ScanGeom make_geom_stale(const RunDesc& d, float4 ps,
     const int ccx = cell_coord(ps.x, i_h), ccy = cell_coord(ps.y, i_h), ccz = cell_coord(ps.z, i_h);
     g.z = ps.z;
     g.slack = 2.0f * disp;
-    const float reach = sqrtf(g.rw2) + disp + 1e-3f * g.cw;
+    const float reach = sqrtf(g.rw2) + disp + g.eps * g.cw;
     g.czlo = max(cell_coord(ps.z - reach, i_h), 0);
     g.czhi = min(cell_coord(ps.z + reach, i_h), g.nz - 1);
     g.edge = ccx <= 0 || ccx >= g.nx - 1 || ccy <= 0 || ccy >= g.ny - 1 || ccz <= 0 || ccz >= g.nz - 1;
@@ -690,7 +697,7 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
     const float* __restrict__ pz = reinterpret_cast<const float*>(pp) + 2;  // pz[4 j] = z of particle j
     const int czlo = g.czlo, czhi = g.czhi;
     const int nl = layer_hi - layer_lo + 1;
-    const float margin = 1e-3f * g.cw;
+    const float margin = g.eps * g.cw;
     // Stale tables: a neighbour's old position may lie one ring of columns
     // further out (read only when some lane's particle is that close to its
     // old cell's side), and one that was outside the grid when the table was
@@ -855,11 +862,11 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
                     const int close = r2 < r2cut ? 1 : 0;
                     const int inside = (j + u < je) ? 1 : 0;
                     hit[u] = close & inside;
-                    // Closer than 0.999 cell (so along every axis), two
-                    // particles' cells differ by at most one (FP32 rounding
-                    // of the cell coordinate is ~1e-5 cell for grids under
-                    // 8000 cells per axis), so only hits at that distance or
-                    // of a lane on the grid's edge need the exact cell test.
+                    // Closer than (1 - eps) cell (so along every axis), two
+                    // particles' cells differ by at most one (the FP32
+                    // rounding of their cell coordinates is well below eps),
+                    // so only hits at that distance or of a lane on the
+                    // grid's edge need the exact cell test.
                     if constexpr (STALE) check |= __ballot(r2 >= g.ext2 && r2 < r2cut);
                 }
                 if constexpr (STALE) {
