@@ -122,6 +122,29 @@ def test_grid_full_size_c2(gpu):
     g.close()
 
 
+@pytest.mark.gpu
+def test_interaction_full_size_c2(gpu):
+    """Interaction on the perturbed 1.55 M-particle column (the benchmark's
+    size): neighbour counts bit-exact and rates within 1e-5 of the oracle."""
+    dev = _dev()
+    import torch
+    case = cases.perturb(cases.column_3d(nsteps=1), seed=61, v_amp=0.3)
+    p, ds, ost = sorted_inputs(case)
+    g, beg, end, ref = _grid_tables(p, ds, ost)
+    rt = dev.DeviceRates(ds.n)
+    nbr = torch.full((ds.n,), -1, dtype=torch.int32, device="cuda")
+    dev.interaction(p, ds, rt, beg, end, nbr)
+    ort, onbr = O.interaction(p, ost, ref["cell_beg"], ref["cell_end"])
+    h = rt.host()
+    m = slice(p.npb, None)
+    assert np.array_equal(nbr.cpu().numpy()[m], onbr[m])
+    for a, k in enumerate(("ax", "ay", "az", "drho")):
+        assert rel_err(h["accad"][m, a], getattr(ort, k)[m]) < TOL, k
+    for c in range(6):
+        assert rel_err(h["acestn"][m, c], ort.d[c, m]) < TOL, c
+    g.close()
+
+
 def _grid_tables(p, ds, ost):
     dev = _dev()
     g = dev.Grid(ds.n, p.ncell)
