@@ -577,6 +577,15 @@ __device__ __forceinline__ bool stale_out_now(const StaleIn& sn, int run) { retu
 // dxy_min is shrunk by eps of a cell (eps = 1e-3, more on grids over ~2000
 // cells per axis), far beyond the FP32 rounding of the coordinates, so every
 // skipped candidate has r2 > r2_cut.
+//
+// STALE: the table was built before the particles moved (make_geom_stale);
+// reach and z-window grow by the run's slack, a ring of columns two cells
+// out and the run's outside cell are read when some lane needs them, and
+// hits that could lie outside the current cell's stencil are tested against
+// it, so the neighbour sets are those of a freshly built table.
+//
+// B = 2 evaluates the list two entries at a time, eval(j0, act0, j1, act1),
+// so that both gathers are in flight before either pair is computed.
 struct ScanGeom {
     int cix, ciy, ciz, nx, ny, nz, nol, cbase;  // stencil cell and grid
     int czlo, czhi;  // z cells read in every column
@@ -635,7 +644,7 @@ __device__ __forceinline__ ScanGeom make_geom(const RunDesc& d, float4 ps) {
 // Geometry for a table built when the particle was at `po` (ps: now), every
 // particle of the run having moved at most `disp` since.  The (x, y) stencil
 // is taken around the old cell; the z cells are those a neighbour's old z can
-// lie in, |z_old - z| <= 1.001 R + disp (plus rounding).
+// lie in, |z_old - z| <= (1 + eps) R + disp (plus rounding).
 __device__ __forceinline__ ScanGeom make_geom_stale(const RunDesc& d, float4 ps, float4 po, float disp, bool out_now) {
     ScanGeom g = make_geom(d, po);
     const float i_h = g.i_h;
@@ -660,9 +669,6 @@ __device__ __forceinline__ bool in_current_stencil(const ScanGeom& g, float4 ps,
            abs(z - cell_coord(ps.z, g.i_h)) <= 1;
 }
 
-//
-// B = 2 evaluates the list two entries at a time, eval(j0, act0, j1, act1),
-// so that both gathers are in flight before either pair is computed.
 template <int B, bool PADDED, int CAP = IC2_CAP, bool STALE = false, class Eval>
 __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer_hi, const ScanGeom& g, float4 ps,
                                             float r2cut, const int32_t* __restrict__ beg,
