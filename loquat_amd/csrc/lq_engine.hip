@@ -360,7 +360,7 @@ struct RunSlot {
 
 struct Buffers {
     DevState S[2];
-    DevIni I;  // written by the predictor, read by the corrector through the sort permutation
+    DevIni I;  // written by the predictor, read by the corrector (and, positions, by the stale scans)
     DevRates R;
     tensor2d* T;
     int32_t* run_of;
@@ -494,8 +494,7 @@ int enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
         }
         {
             StageTimer t(prof, s, PG_INTEG);
-            launch_integrate_table(corr, b.desc, b.run_of, b.S[c], b.I, b.R, mats, nm, nullptr, b.fail, b.dmax, b.nout,
-                                   s);
+            launch_integrate_table(corr, b.desc, b.run_of, b.S[c], b.I, b.R, mats, nm, b.fail, b.dmax, b.nout, s);
             if (!corr) launch_dmax_fold(b.dmax, nr, s);
         }
     }

@@ -99,16 +99,15 @@ void launch_interaction_table(const RunDesc* d, const int32_t* run_of, const Dev
                               int32_t* nbr, int variant, const StaleIn* stale, hipStream_t s);
 void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, const DevIni& ini,
                           const DevRates& rt, hipStream_t s);
-// ini_perm: the corrector reads particle k's ini entry at ini_perm[k] (the
-// last sort's permutation; nullable = identity).  fail: per-run flags raised
-// by the corrector on a non-finite state (nullable).  dmax (nullable, per
+// fail: per-run flags raised by the corrector on a non-finite state
+// (nullable).  dmax (nullable, per
 // run, zeroed by launch_step_begin): the predictor records each run's largest
 // displacement (and nout, per run, whether a particle ended outside the
 // grid); the corrector raises fail when the displacement exceeded what a
 // stale table tolerates.
 void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
-                            const DevIni& ini, const DevRates& rt, IndexMap mats, int nmat, const uint32_t* ini_perm,
-                            int32_t* fail, uint32_t* dmax, uint32_t* nout, hipStream_t s);
+                            const DevIni& ini, const DevRates& rt, IndexMap mats, int nmat, int32_t* fail,
+                            uint32_t* dmax, uint32_t* nout, hipStream_t s);
 // Shepard filter: reads stress from str_src, writes str_dst (walls and runs
 // not due are copied through).
 void launch_shepard_one(const RunDesc& d, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,

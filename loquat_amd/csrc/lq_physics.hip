@@ -1356,17 +1356,15 @@ __global__ void k_dmax_fold(uint32_t* __restrict__ dmax, int nruns) {
     dmax[r * DMAX_SLOTS] = m;
 }
 
-// Corrector (src/predictor_corrector.cu:256-307): midpoint update from ini.
-// ini_perm (nullable): the ini set is still in the order of the previous
-// sort, and particle k's entry is ini[ini_perm[k]] -- the reference permutes
-// all five *_ini arrays instead (SortArraysAll, src/loquat.cu:200-213); the
-// values read are identical.
+// Corrector (src/predictor_corrector.cu:256-307): midpoint update from ini,
+// which is in the order of the state (no neighbour search separates the
+// predictor from the corrector, so nothing needs the reference's
+// SortArraysAll of the *_ini arrays, src/loquat.cu:200-213).
 // fail (nullable, one flag per run) is raised when a particle's new state is
 // not finite, so a diverging run can be stopped without touching the others.
 template <class DP>
 __global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni ini, DevRates rt, IndexMap mats,
-                                                   int n, const uint32_t* __restrict__ ini_perm,
-                                                   int32_t* __restrict__ fail,
+                                                   int n, int32_t* __restrict__ fail,
                                                    const uint32_t* __restrict__ dmax = nullptr) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -1374,25 +1372,24 @@ __global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni in
     const int run = dp.run(k);
     const RunDesc& d = dp.get(run);
     if ((k - d.p_off) < d.npb || !d.active) return;
-    const int q = ini_perm ? (int)ini_perm[k] : k;
     const float dt = d.par.dt;
     const double dtd = (double)dt;
     const float4 a = rt.accad[k];
-    const float4 v0 = ini.velrhop[q];
+    const float4 v0 = ini.velrhop[k];
     float4 vr;
     vr.x = v0.x + dt * a.x;
     vr.y = v0.y + dt * a.y;
     vr.z = v0.z + dt * a.z;
     vr.w = v0.w + dt * a.w;
-    const double x = 0.5 * ((double)vr.x + (double)v0.x) * dtd + ini.xd[q];
-    const double y = 0.5 * ((double)vr.y + (double)v0.y) * dtd + ini.yd[q];
-    const double z = 0.5 * ((double)vr.z + (double)v0.z) * dtd + ini.zd[q];
+    const double x = 0.5 * ((double)vr.x + (double)v0.x) * dtd + ini.xd[k];
+    const double y = 0.5 * ((double)vr.y + (double)v0.y) * dtd + ini.yd[k];
+    const double z = 0.5 * ((double)vr.z + (double)v0.z) * dtd + ini.zd[k];
     const tensor2d D = rt.acestn[k];
-    const tensor2d stn = tadd(ini.stn[q], tscale(D, dt));
+    const tensor2d stn = tadd(ini.stn[k], tscale(D, dt));
     float p = st.pospres[k].w;
-    const float2 epvr = ini.epvr[q];
+    const float2 epvr = ini.epvr[k];
     float ep = epvr.x, voidr = epvr.y;
-    tensor2d s = ini.str[q];
+    tensor2d s = ini.str[k];
     constitutive(d, dt, rt.spin[k], D, stn, p, ep, voidr, s);
     if (fail) {
         const float chk = (vr.x + vr.y + vr.z + vr.w) + (s.xx + s.yy + s.zz) + (s.xy + s.xz + s.yz);
@@ -1680,18 +1677,17 @@ void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, 
     if (n <= 0) return;
     const IndexMap mats{nullptr, d.npb};
     if (corrector)
-        k_corrector<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, mats, n, nullptr, nullptr);
+        k_corrector<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, mats, n, nullptr);
     else
         k_predictor<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, mats, n, nullptr, nullptr);
 }
 void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
-                            const DevIni& ini, const DevRates& rt, IndexMap mats, int nmat, const uint32_t* ini_perm,
-                            int32_t* fail, uint32_t* dmax, uint32_t* nout, hipStream_t s) {
+                            const DevIni& ini, const DevRates& rt, IndexMap mats, int nmat, int32_t* fail,
+                            uint32_t* dmax, uint32_t* nout, hipStream_t s) {
     if (nmat <= 0) return;
     const int nb = blocks_for(nmat, 256);
     if (corrector)
-        k_corrector<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat, ini_perm, fail,
-                                                  dmax);
+        k_corrector<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat, fail, dmax);
     else
         k_predictor<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat, dmax, nout);
 }
