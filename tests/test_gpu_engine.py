@@ -438,3 +438,34 @@ def test_engine_edge_cases_and_errors(gpu):
         small.add(normal, 1)
     assert err.value.status == 8
     small.close()
+
+
+@pytest.mark.gpu
+def test_run_moving_beyond_the_table_reach_is_stopped(gpu):
+    """A run whose particles move more than 0.45 cell in half a step (a
+    uniform 1000 m/s translation: no strain rate, so the state stays finite)
+    is flagged failed by the corrector, because the corrector half's reuse of
+    the cell table would no longer be exact; a run beside it is unaffected."""
+    import ctypes as C
+    good = gentle_case(1, seed=71)
+    fast = gentle_case(1, seed=72)
+    npb = fast.par.npb
+    fast.velrhop[npb:, 0] = 1000.0
+    p = fast.derived()
+    assert 1000.0 * 0.5 * p.dt > 0.45 * (2.0 / p.i_h)
+    e, (rg, rf) = _engine([good, fast], 5)
+    e.step(5)
+    e.sync()
+    lib = abi.lib()
+    nfailed = C.c_int32()
+    abi.check(lib.lq_engine_poll(e.h, C.byref(nfailed)))
+    fg, ff = C.c_int32(), C.c_int32()
+    abi.check(lib.lq_engine_run_failed(e.h, rg, C.byref(fg)))
+    abi.check(lib.lq_engine_run_failed(e.h, rf, C.byref(ff)))
+    assert nfailed.value == 1 and ff.value == 1 and fg.value == 0
+    got = e.download(rg)
+    e.close()
+    e2, (r2,) = _engine([good], 5)
+    e2.step(5)
+    assert _same(got, e2.download(r2))
+    e2.close()
