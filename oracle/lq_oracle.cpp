@@ -687,6 +687,41 @@ void lqo_constitutive(const lq_params* par, const lq_vw_params* vw, double dt, c
     s6[5] = s.yz;
 }
 
+// Pairs whose neighbour decision depends on the precision of the predicate
+// (an oracle diagnostic, no reference counterpart): over the material
+// particles' 27-cell stencils (both layers, j != i), the reference's FP32
+// test is_nbr against q = r / h < 2 evaluated in FP64 on the FP64 positions.
+// out[0]: FP32 accepts and FP64 rejects; out[1]: FP64 accepts and FP32
+// rejects.  Counts are per ordered pair (a material-material pair counts
+// from both sides).
+void lqo_borderline_pairs(const lq_params* par, const lqo_state* st, const int32_t* beg, const int32_t* end,
+                          int64_t* out) {
+    const Grid g = grid_of(par);
+    const float i_hf = par->i_h;
+    const double i_h = par->i_h;
+    const int n = par->np;
+    int64_t only32 = 0, only64 = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : only32, only64)
+    for (int i = par->npb; i < n; ++i) {
+        const float xf = (float)st->x[i], yf = (float)st->y[i], zf = (float)st->z[i];
+        const int cix = cell_coord(xf, i_hf), ciy = cell_coord(yf, i_hf), ciz = cell_coord(zf, i_hf);
+        for (int layer = 0; layer <= 1; ++layer) {
+            stencil(g, cix, ciy, ciz, layer, beg, end, [&](int b, int e) {
+                for (int j = b; j < e; ++j) {
+                    if (j == i) continue;
+                    const bool f32 = is_nbr(xf, yf, zf, (float)st->x[j], (float)st->y[j], (float)st->z[j], i_hf);
+                    const double dx = st->x[i] - st->x[j], dy = st->y[i] - st->y[j], dz = st->z[i] - st->z[j];
+                    const bool f64 = std::sqrt(dx * dx + dy * dy + dz * dz) * i_h < 2.0;
+                    only32 += (f32 && !f64) ? 1 : 0;
+                    only64 += (f64 && !f32) ? 1 : 0;
+                }
+            });
+        }
+    }
+    out[0] = only32;
+    out[1] = only64;
+}
+
 void lqo_set_num_threads(int n) {
 #ifdef _OPENMP
     if (n > 0) omp_set_num_threads(n);

@@ -49,6 +49,7 @@ def lib() -> C.CDLL:
         L.lqo_permute.argtypes = [C.c_int, ip, C.POINTER(_State), C.POINTER(_Ini)]
         L.lqo_adami.argtypes = [vp, C.POINTER(_State), ip, ip]
         L.lqo_interaction.argtypes = [vp, C.POINTER(_State), C.POINTER(_Rates), ip, ip, ip]
+        L.lqo_borderline_pairs.argtypes = [vp, C.POINTER(_State), ip, ip, C.POINTER(C.c_int64)]
         L.lqo_predictor.argtypes = [vp, vp, C.POINTER(_State), C.POINTER(_Ini), C.POINTER(_Rates)]
         L.lqo_corrector.argtypes = [vp, vp, C.POINTER(_State), C.POINTER(_Ini), C.POINTER(_Rates)]
         L.lqo_shepard.argtypes = [vp, C.POINTER(_State), ip, ip]
@@ -234,6 +235,15 @@ def interaction(par, st: OState, beg, end) -> tuple[ORates, np.ndarray]:
     s, r = st.struct(), rt.struct()
     lib().lqo_interaction(_p(par), C.byref(s), C.byref(r), _i(beg), _i(end), _i(nbr))
     return rt, nbr
+
+
+def borderline_pairs(par, st: OState, beg, end) -> tuple[int, int]:
+    """(pairs the FP32 predicate accepts and FP64 rejects, the reverse),
+    ordered pairs over the material particles' stencils."""
+    out = (C.c_int64 * 2)()
+    s = st.struct()
+    lib().lqo_borderline_pairs(_p(par), C.byref(s), _i(beg), _i(end), out)
+    return int(out[0]), int(out[1])
 
 
 def predictor(par, vw, st: OState, ini: OIni, rt: ORates) -> None:

@@ -182,6 +182,27 @@ def _two_particles(u: float):
     return c
 
 
+def test_borderline_pairs_follow_the_fp32_predicate():
+    """Two particles 2h (1 + 5e-8) apart: the FP32 test of the reference
+    accepts them (their FP32 coordinates differ by less than 2h), FP64 on the
+    FP64 positions rejects them.  The neighbour count follows FP32 and the
+    borderline diagnostic reports the pair from both sides; 1e-6 closer, both
+    agree."""
+    for rel, flips in ((5e-8, 2), (-1e-6, 0)):
+        c = _two_particles(0.0)
+        p = c.derived()
+        d = (2.0 / float(np.float32(p.i_h))) * (1.0 + rel)
+        c.posd[1, 0] = c.posd[0, 0] + d
+        c.pospres[1, 0] = np.float32(c.posd[1, 0])
+        st = O.OState.from_case(c)
+        g = O.grid_build(p, st)
+        O.permute(g["part_idx"], st)
+        g = O.grid_build(p, st)
+        _, nbr = O.interaction(p, st, g["cell_beg"], g["cell_end"])
+        assert list(nbr) == [1, 1]
+        assert O.borderline_pairs(p, st, g["cell_beg"], g["cell_end"]) == (flips, 0)
+
+
 def test_viscosity_dissipates_approach_only():
     for u, approaching in ((0.2, True), (-0.2, False)):
         c = _two_particles(u)
