@@ -89,10 +89,13 @@ def _base_params(**kw) -> abi.LqParams:
 
 def build_case(name: str, mat: np.ndarray, walls: np.ndarray, *, dx: float, domain: tuple, rho0: float,
                model: int, mpar: tuple, cs: float, hdr: float = 1.2, cfl: float = 0.1, alpha: float = 0.25,
-               shepard: int = 0, nsteps: int = 100, g: float = -9.81, z_top: float | None = None,
+               shepard: int = 0, nsteps: int = 100, g: float = -9.81, z_top: float | np.ndarray | None = None,
                k0: float = 0.5, vw: abi.LqVw | None = None, e0: np.ndarray | None = None,
                time_max: float | None = None) -> Case:
-    """Assembles a project: walls first (indices [0, npb)), then material."""
+    """Assembles a project: walls first (indices [0, npb)), then material.
+    z_top (scalar, or one value per material particle): height of the free
+    surface above each particle, for the geostatic initial stress (vertical
+    rho0 g (z_top - z), horizontal k0 times that)."""
     npb, npm = len(walls), len(mat)
     n = npb + npm
     posd = np.concatenate([walls, mat]).astype(np.float64)
@@ -234,10 +237,13 @@ def slope_vw(npm_target: float = 4.0e6, nsteps: int = 10000, seed: int = 0, dept
         _lattice(nbx, LAYERS, wall_h, o, 0.0, o, dx),
         _lattice(nbx, LAYERS, wall_h, o, o + depth, o, dx),
     ])
+    # geostatic stress under the local surface: the crest, or the slope face
+    # above the particle
+    z_surf = o + np.minimum(height, (length - (mat[:, 0] - o)) * math.tan(math.radians(angle)))
     vw = abi.LqVw(**VW_SAND)
     case = build_case("slope_vw", mat, walls, dx=dx, domain=(o + box_x + dx, 2 * o + depth, o + 1.3 * height),
                       rho0=rho0, model=3, mpar=(0, 0, 0, 0, 0), cs=cs, shepard=0, nsteps=nsteps,
-                      z_top=o + height, k0=0.5, vw=vw, e0=ev)
+                      z_top=z_surf, k0=0.5, vw=vw, e0=ev)
     case.meta.update(dict(dx=dx, floor_z=o, wall_x=o))
     return case
 

@@ -1197,6 +1197,16 @@ __device__ __forceinline__ void mat_vw(const RunDesc& d, float dt, const tensor2
     rate = tscale(rate, fs / tt);
     s = tadd(s, tscale(tadd(jau, rate), dt));
     e = e + dt * (1.0f + e) * (D.xx + D.yy + D.zz);
+    // A cohesionless material carries no tensile mean stress: such a state
+    // returns to zero stress (what the Drucker-Prager tension cut and apex
+    // return give for c = 0).  The void ratio stays between the densest and
+    // the loosest state at the new pressure, e_d <= e <= e_i.  Both tests
+    // let a NaN through, for the corrector's failure check.
+    const float trn = s.xx + s.yy + s.zz;
+    if (trn >= 0.0f) s = tensor2d{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const float exn = expf(-powf(fmaxf(-trn, 0.0f) / hs, nn));
+    const float elo = d.vw.ed0 * exn, ehi = d.vw.ei0 * exn;
+    e = e < elo ? elo : (e > ehi ? ehi : e);
 }
 
 __device__ __forceinline__ void constitutive(const RunDesc& d, float dt, const float4 w, const tensor2d& D,

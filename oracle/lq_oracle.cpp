@@ -270,6 +270,14 @@ void mat_vw(const lq_vw_params* vw, double dt, const T6& D, double wx, double wy
     rate = mul(rate, fs / tt);
     s = add(s, mul(add(jau, rate), dt));
     e = e + dt * (1.0 + e) * (D.xx + D.yy + D.zz);
+    // Tension cut (zero stress for a tensile mean stress, as the DP tension
+    // cut plus apex return give for c = 0) and e_d <= e <= e_i at the new
+    // pressure; NaN passes both.
+    const double trn = s.xx + s.yy + s.zz;
+    if (trn >= 0.0) s = T6{0, 0, 0, 0, 0, 0};
+    const double exn = std::exp(-std::pow(std::max(-trn, 0.0) / hs, n));
+    const double elo = vw->ed0 * exn, ehi = vw->ei0 * exn;
+    e = e < elo ? elo : (e > ehi ? ehi : e);
 }
 
 void constitutive(const lq_params* par, const lq_vw_params* vw, double dt, double wx, double wy, double wz,

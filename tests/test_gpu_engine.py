@@ -469,3 +469,30 @@ def test_run_moving_beyond_the_table_reach_is_stopped(gpu):
     e2.step(5)
     assert _same(got, e2.download(r2))
     e2.close()
+
+
+@pytest.mark.gpu
+def test_slope_von_wolffersdorff_stays_stable(gpu):
+    """The von Wolffersdorff embankment at 0.3 M particles runs 3,000 steps
+    (0.105 s, three times past where the free-surface tension used to set
+    off a blow-up at the toe) without a failure flag: finite state, void
+    ratio inside [e_d0, e_i0], no tensile mean stress, particles slower than
+    5 m/s."""
+    import ctypes as C
+    from loquat_amd import abi
+    case = cases.slope_vw(npm_target=2e5, nsteps=3000)
+    e, (rid,) = _engine([case], 3000)
+    e.step(3000)
+    e.sync()
+    nfailed = C.c_int32()
+    abi.check(abi.lib().lq_engine_poll(e.h, C.byref(nfailed)))
+    info = e.info(rid)
+    h = e.download(rid)
+    e.close()
+    assert nfailed.value == 0 and info["finished"] and info["steps"] == 3000
+    m = slice(case.par.npb, None)
+    vel, s, ev = h["velrhop"][m], h["str"][m], h["idep"][m, 2]
+    assert np.isfinite(h["posd"][m]).all() and np.isfinite(vel).all() and np.isfinite(s).all()
+    assert np.abs(vel[:, :3]).max() < 5.0
+    assert (s[:, :3].sum(1) <= 0).all()
+    assert ev.min() >= case.vw.ed0 * (1 - 1e-6) and ev.max() <= case.vw.ei0 * (1 + 1e-6)

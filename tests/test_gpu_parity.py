@@ -54,11 +54,23 @@ def model_case(model: int, seed: int = 3):
     if model == 2:
         c.par.constitutive_model = 2
         (c.par.model_par_1, c.par.model_par_2, c.par.model_par_3, c.par.model_par_4, c.par.model_par_5) = WU
-    elif model == 3:
+    elif model in (3, "3-bounds"):
         c.par.constitutive_model = 3
         c.vw = abi.LqVw(**cases.VW_SAND)
         rng = np.random.default_rng(seed)
         c.idep[c.par.npb:, 2] = (0.7 * (1 + rng.uniform(-0.05, 0.05, c.npm))).astype(np.float32)
+        if model == "3-bounds":
+            # a tenth of the material nearly stress-free (the tension cut
+            # where the rates extend it), a tenth with e just outside
+            # [e_d, e_i] (about [0.58, 1.05] at these pressures).  Small
+            # subsets, so that the fields' norms, against which the FP32
+            # error is measured, stay those of the ordinary particles.
+            k = c.par.npb + rng.permutation(c.npm)
+            tenth = c.npm // 10
+            low = k[:tenth]
+            c.str[low, :3] = -2.0
+            c.str[low, 3:] = 0.5
+            c.idep[k[tenth:2 * tenth], 2] = rng.choice([0.55, 1.2], tenth).astype(np.float32)
     return c
 
 
@@ -266,8 +278,10 @@ def _check_state(p, h, ost, before, tol=TOL):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("model", [1, 2, 3])
+@pytest.mark.parametrize("model", [1, 2, 3, "3-bounds"])
 def test_predictor_corrector_parity(gpu, model):
+    """One predictor and one corrector from the same rates; "3-bounds" makes
+    von Wolffersdorff's tension cut and void-ratio bounds act."""
     dev = _dev()
     case = model_case(model)
     p, ds, ost = sorted_inputs(case)
@@ -281,6 +295,14 @@ def test_predictor_corrector_parity(gpu, model):
     O.predictor(p, case.vw, ost, oini, ort)
     h = ds.host()
     _check_state(p, h, ost, before)
+    if model == "3-bounds":
+        m = slice(p.npb, None)
+        tr = ost.s[0, m] + ost.s[1, m] + ost.s[2, m]
+        exn = np.exp(-(np.maximum(-tr, 0) / case.vw.hs) ** case.vw.n)
+        ev = ost.voidr[m]
+        assert (np.abs(ost.s[:, m]).max(0) == 0).sum() > 20          # tension cut
+        at = lambda e0: np.isclose(ev, e0 * exn, rtol=1e-12, atol=0).sum()
+        assert at(case.vw.ed0) > 20 and at(case.vw.ei0) > 20              # void-ratio bounds
     # corrector from the predicted state and the same rates
     dev.corrector(p, case.vw, ds, ini, rt)
     O.corrector(p, case.vw, ost, oini, ort)

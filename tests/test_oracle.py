@@ -462,6 +462,48 @@ def test_vw_isotropic_compression_follows_bauer_law():
     assert e == pytest.approx(ei(p), rel=2e-3)
 
 
+def _vw_par():
+    par = abi.LqParams()
+    par.constitutive_model = 3
+    return par, abi.LqVw(**cases.VW_SAND)
+
+
+def test_vw_tension_cut_gives_zero_stress_like_dp():
+    """Von Wolffersdorff under an extension that makes the mean stress
+    tensile returns to zero stress, as Drucker-Prager with c = 0 does (its
+    tension cut plus apex return); a compressive result is left alone."""
+    par, vw = _vw_par()
+    S = [-50.0, -40.0, -30.0, 5.0, -3.0, 2.0]
+    D = [2e-3, 1e-3, 1.5e-3, 4e-4, 0.0, -2e-4]
+    s, p, _, _ = O.constitutive(par, vw, 1.0, [0.1, 0.0, -0.2], D, S, voidr=0.7)
+    assert np.all(s == 0.0) and p == 0.0
+    sd, _, _, _ = O.constitutive(dp_params(phi=30.0, c=0.0), None, 1.0, [0.1, 0.0, -0.2], D, S)
+    assert np.all(sd == 0.0)
+    # mild compression stays compressive and is not cut
+    s, p, _, _ = O.constitutive(par, vw, 1.0, [0, 0, 0], [-1e-6] * 3 + [0] * 3, S, voidr=0.7)
+    assert s[0] + s[1] + s[2] < -120.0 and p > 40.0
+
+
+def test_vw_void_ratio_held_between_densest_and_loosest():
+    """e after the update lies in [e_d0, e_i0] exp(-(3p/h_s)^n) at the new
+    mean pressure p (Bauer's law for the densest and loosest states); NaN
+    is passed on, not clamped."""
+    par, vw = _vw_par()
+    bound = lambda e0, s: e0 * math.exp(-(-(s[0] + s[1] + s[2]) / vw.hs) ** vw.n)
+    S = [-2e4, -1.5e4, -1e4, 0, 0, 0]
+    # strong compaction from the densest state: e would fall below e_d
+    s, _, _, e = O.constitutive(par, vw, 1.0, [0, 0, 0], [-0.05, -0.05, -0.05, 0, 0, 0], S, voidr=0.62)
+    assert s[0] + s[1] + s[2] < 0 and e == pytest.approx(bound(vw.ed0, s), rel=1e-12)
+    # a loose start above e_i is brought down to e_i at the new pressure
+    s, _, _, e = O.constitutive(par, vw, 1e-4, [0, 0, 0], [0.0] * 6, S, voidr=1.5)
+    assert e == pytest.approx(bound(vw.ei0, s), rel=1e-12)
+    # inside the bounds the void ratio follows de = (1 + e) tr(D) dt
+    s, _, _, e = O.constitutive(par, vw, 1.0, [0, 0, 0], [-1e-4, 0, 0, 0, 0, 0], S, voidr=0.7)
+    assert e == pytest.approx(0.7 - 1.7e-4, rel=1e-12)
+    _, _, _, e = O.constitutive(par, vw, 1.0, [0, 0, 0], [0.0] * 6, S, voidr=float("nan"))
+    assert math.isnan(e)
+
+
 def test_midpoint_integration_exact_for_constant_acceleration():
     case = lattice_case(nx=6, ny=6, nz=6, walls=False)
     p = case.derived()
