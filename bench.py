@@ -152,20 +152,23 @@ def ensemble_section(nruns: int, steps: int, warm: int, seed: int = 0) -> dict:
     npm_tot = sum(c.npm for c in runs)
     # batched
     eng = Engine(sum(c.np for c in runs), sum(c.derived().ncell for c in runs), len(runs))
-    for c in runs:
-        eng.add(c)
+    ids = [eng.add(c) for c in runs]
     eng.step(warm)
     eng.sync()
     t0 = time.perf_counter()
     eng.step(steps)
     eng.sync()
     t_batch = time.perf_counter() - t0
+    # timed work actually done (a failed run stops early)
+    done = [eng.info(r)["steps"] for r in ids]
+    work = sum(c.npm * max(0, k - warm) for c, k in zip(runs, done))
+    failed = sum(k < steps + warm for k in done)
     eng.close()
     torch.cuda.empty_cache()
     # plain loop: one run at a time, each a batch of one
     big = max(runs, key=lambda c: c.np)
     eng = Engine(big.np, max(c.derived().ncell for c in runs), 1)
-    per_run, completion, clock = [], [], 0.0
+    per_run, completion, clock, work_plain = [], [], 0.0, 0
     for c in runs:
         rid = eng.add(c)
         eng.step(warm)
@@ -174,13 +177,14 @@ def ensemble_section(nruns: int, steps: int, warm: int, seed: int = 0) -> dict:
         eng.step(steps)
         eng.sync()
         dt = time.perf_counter() - t0
+        work_plain += c.npm * max(0, eng.info(rid)["steps"] - warm)
         eng.remove(rid)
         per_run.append(dt)
         clock += dt
         completion.append(clock)
     eng.close()
-    batched = npm_tot * steps / t_batch / 1e6
-    plain = npm_tot * steps / sum(per_run) / 1e6
+    batched = work / t_batch / 1e6
+    plain = work_plain / sum(per_run) / 1e6
     return {
         "workload": f"{nruns} independent 3D column collapses, npm log-uniform 2k..60k (sum {npm_tot}), "
                     f"DP phi U[20,40] deg, c U[0,5] kPa, Shepard every 20 steps",
@@ -189,6 +193,7 @@ def ensemble_section(nruns: int, steps: int, warm: int, seed: int = 0) -> dict:
         "batched_M_particle_steps_s": round(batched, 3),
         "plain_loop_M_particle_steps_s": round(plain, 3),
         "speedup_vs_plain_loop": round(batched / plain, 3),
+        "failed_runs": failed,
         "batched_completion_s": {"p50": round(t_batch, 4), "p99": round(t_batch, 4)},
         "plain_loop_completion_s": {"p50": round(_pct(completion, 50), 4), "p99": round(_pct(completion, 99), 4)},
     }
@@ -227,7 +232,7 @@ def served_section(nreq: int, step_scale: float, seed: int = 0) -> dict:
     np_cap = sum(c.np for _, c, _ in reqs)
     nc_cap = sum(c.derived().ncell for _, c, _ in reqs)
     srv = Server(np_cap, nc_cap, 256, 288 * 10**9, steps_per_tick=25)
-    done_at, ids = {}, {}
+    done_at, ids, fin = {}, {}, {}
     t0 = time.perf_counter()
     nxt = 0
     while len(done_at) < nreq:
@@ -241,15 +246,19 @@ def served_section(nreq: int, step_scale: float, seed: int = 0) -> dict:
             continue
         for rid in srv.tick():
             done_at[ids[rid]] = time.perf_counter() - t0
+            fin[ids[rid]] = srv.info(rid)
     makespan = time.perf_counter() - t0
     srv.close()
     torch.cuda.empty_cache()
     lat = [done_at[i] - reqs[i][0] for i in range(nreq)]
-    work = sum(c.npm * s for _, c, s in reqs)
+    # work actually done: a run that fails stops early and is credited only
+    # with the steps it took
+    work = sum(reqs[i][1].npm * fin[i]["steps"] for i in range(nreq))
+    failed = sum(fin[i]["state"] != "done" for i in range(nreq))
     # plain loop: FIFO, one run at a time on the same engine code
     big = max(reqs, key=lambda r: r[1].np)[1]
     eng = Engine(big.np, max(c.derived().ncell for _, c, _ in reqs), 1)
-    clock, lat_plain, busy = 0.0, [], 0.0
+    clock, lat_plain, busy, work_plain = 0.0, [], 0.0, 0
     for a, c, s in reqs:
         rid = eng.add(c, s)
         eng.sync()
@@ -257,6 +266,7 @@ def served_section(nreq: int, step_scale: float, seed: int = 0) -> dict:
         eng.step(s)
         eng.sync()
         dt = time.perf_counter() - t1
+        work_plain += c.npm * eng.info(rid)["steps"]
         eng.remove(rid)
         busy += dt
         clock = max(clock, a) + dt
@@ -268,7 +278,8 @@ def served_section(nreq: int, step_scale: float, seed: int = 0) -> dict:
         "latency_s": {"p50": round(_pct(lat, 50), 4), "p99": round(_pct(lat, 99), 4)},
         "plain_loop_latency_s": {"p50": round(_pct(lat_plain, 50), 4), "p99": round(_pct(lat_plain, 99), 4)},
         "aggregate_M_particle_steps_s": round(work / makespan / 1e6, 3),
-        "plain_loop_M_particle_steps_s": round(work / busy / 1e6, 3),
+        "plain_loop_M_particle_steps_s": round(work_plain / busy / 1e6, 3),
+        "failed_runs": failed,
         "makespan_s": round(makespan, 3),
     }
 
@@ -283,7 +294,7 @@ def slope_section(steps: int, npm_target: float = 4.0e6) -> dict:
 
     c = cases.slope_vw(npm_target=npm_target, nsteps=steps + 5)
     eng = Engine(c.np, c.derived().ncell, 1)
-    eng.add(c)
+    rid = eng.add(c)
     eng.step(5)
     eng.sync()
     s = torch.cuda.ExternalStream(eng.stream())
@@ -293,7 +304,10 @@ def slope_section(steps: int, npm_target: float = 4.0e6) -> dict:
     e1.record(s)
     eng.sync()
     ms = e0.elapsed_time(e1)
+    done = eng.info(rid)["steps"] - 5
     eng.close()
+    if done != steps:
+        raise SystemExit(f"bench: the embankment ran {done} of {steps} timed steps (the run failed)")
     torch.cuda.empty_cache()
     return {"workload": f"30-degree embankment 1.0 x 3.0 x 0.5 m, npm={c.npm}, np={c.np}, "
                         f"dx={c.meta['dx'] * 1e3:.2f} mm, von Wolffersdorff",
@@ -374,6 +388,9 @@ def main() -> None:
     gpu_ms = ev0.elapsed_time(ev1)
     ms_per_step = gpu_ms / a.steps
     log(f"timed {a.steps} steps: {gpu_ms:.2f} ms on the engine stream, {wall * 1e3:.2f} ms wall")
+    if eng.info(rid)["steps"] != a.warmup + a.steps:
+        raise SystemExit(f"bench: the column ran {eng.info(rid)['steps']} of {a.warmup + a.steps} steps "
+                         "(the run failed); no throughput to report")
 
     # Per-stage device time (HIP events around each stage, direct launches,
     # same stream and state, right after the timed loop).
