@@ -1150,6 +1150,7 @@ __device__ __forceinline__ void mat_wu(const RunDesc& d, float dt, const tensor2
 // of Herle & Gudehus 1999).  Tension-positive stress; p = -tr(T)/3 is floored
 // at VW_PMIN for the rate evaluation.  Updates stress and void ratio.
 constexpr float VW_PMIN = 1.0f;
+constexpr float VW_TPSI_CAP = 1.4142135f * (1.0f - 1e-3f);
 
 __device__ __forceinline__ void mat_vw(const RunDesc& d, float dt, const tensor2d& D, const float4 w, tensor2d& s,
                                        float& e) {
@@ -1178,8 +1179,13 @@ __device__ __forceinline__ void mat_vw(const RunDesc& d, float dt, const tensor2
         c3t = -2.4494897f * 3.0f * det / (ss * sqrtf(ss));
         c3t = fminf(fmaxf(c3t, -1.0f), 1.0f);
     }
-    const float farg = tpsi * tpsi / 8.0f + (2.0f - tpsi * tpsi) / (2.0f + 1.4142135f * tpsi * c3t);
-    const float F = sqrtf(fmaxf(farg, 0.0f)) - tpsi / (2.0f * 1.4142135f);
+    // F with tan(psi) capped just below sqrt(2): a stress without a tensile
+    // principal value has tan(psi) <= sqrt(2) (T / tr T is then positive
+    // semidefinite with unit trace), and beyond it the denominator
+    // 2 + sqrt(2) tan(psi) cos(3 theta) can vanish, which makes F unbounded.
+    const float tpf = fminf(tpsi, VW_TPSI_CAP);
+    const float farg = tpf * tpf / 8.0f + (2.0f - tpf * tpf) / (2.0f + 1.4142135f * tpf * c3t);
+    const float F = sqrtf(fmaxf(farg, 0.0f)) - tpf / (2.0f * 1.4142135f);
     const float a = d.vw_a;
     const float hs = d.vw.hs, nn = d.vw.n;
     const float x = 3.0f * p / hs;

@@ -247,8 +247,11 @@ void mat_vw(const lq_vw_params* vw, double dt, const T6& D, double wx, double wy
         c3t = -std::sqrt(6.0) * 3.0 * det / (ss * std::sqrt(ss));
         c3t = std::min(1.0, std::max(-1.0, c3t));
     }
-    const double farg = tpsi * tpsi / 8.0 + (2.0 - tpsi * tpsi) / (2.0 + std::sqrt(2.0) * tpsi * c3t);
-    const double F = std::sqrt(std::max(farg, 0.0)) - tpsi / (2.0 * std::sqrt(2.0));
+    // tan(psi) capped just below sqrt(2), the bound of stresses without a
+    // tensile principal value, where the denominator can vanish
+    const double tpf = std::min(tpsi, (double)(1.4142135f * (1.0f - 1e-3f)));
+    const double farg = tpf * tpf / 8.0 + (2.0 - tpf * tpf) / (2.0 + std::sqrt(2.0) * tpf * c3t);
+    const double F = std::sqrt(std::max(farg, 0.0)) - tpf / (2.0 * std::sqrt(2.0));
     const double sp = std::sin((double)vw->phi_c * pi / 180.0);
     const double a = std::sqrt(3.0) * (3.0 - sp) / (2.0 * std::sqrt(2.0) * sp);
     const double hs = vw->hs, n = vw->n;

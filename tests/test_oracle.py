@@ -504,6 +504,30 @@ def test_vw_void_ratio_held_between_densest_and_loosest():
     assert math.isnan(e)
 
 
+def test_vw_stiffness_bounded_outside_the_no_tension_region():
+    """Von Wolffersdorff's F has the denominator 2 + sqrt(2) tan(psi)
+    cos(3 theta), which can vanish once tan(psi) > sqrt(2) (a state with a
+    tensile principal stress).  At such a state -- pressure 905 Pa, shear
+    3.3 kPa, denominator -1.6e-4 uncapped -- a strain increment of 1.3e-4
+    moves the stress by less than 100 Pa; and approaching uniaxial
+    compression (tan(psi) -> sqrt(2), cos(3 theta) = -1) the update is
+    continuous."""
+    par, vw = _vw_par()
+    S = [-2121.4392, 186.51776, -780.41656, 3270.0193, -1282.4542, 696.94556]
+    D = [-18.57, 0.4588, -11.64, -4.426, -7.634, 7.994]
+    s, _, _, _ = O.constitutive(par, vw, 6.4e-6, [0, 0, 0], D, S, voidr=0.71)
+    assert np.abs(np.asarray(s) - S).max() < 100.0
+    Du = [-1e-3, 2e-4, 3e-4, 1e-4, 0, 0]
+    out = []
+    for lam in (1e-2, 1e-3, 1e-4, 1e-5, 0.0):
+        S = [-1e4, -1e4 * lam, -1e4 * lam, 0, 0, 0]
+        s, _, _, _ = O.constitutive(par, vw, 1e-3, [0, 0, 0], Du, S, voidr=0.7)
+        out.append(np.asarray(s) - S)
+    assert np.all(np.isfinite(out))
+    jumps = [np.abs(out[i + 1] - out[i]).max() for i in range(len(out) - 1)]
+    assert max(jumps[1:]) <= jumps[0] and jumps[-1] < 0.05 * np.abs(out[-1]).max()
+
+
 def test_midpoint_integration_exact_for_constant_acceleration():
     case = lattice_case(nx=6, ny=6, nz=6, walls=False)
     p = case.derived()
