@@ -73,7 +73,11 @@ typedef struct lq_params {
 
 /* Constants of constitutive model 3 (von Wolffersdorff hypoplasticity).  The
  * reference has no such model; its five model_par slots cannot hold eight
- * constants.  Angles in degrees, hs in Pa. */
+ * constants.  Angles in degrees, hs in Pa.  The update evaluates the rate
+ * with p floored at 1 Pa and tan(psi) capped just below sqrt(2) (the bound
+ * of stresses without a tensile principal value), returns a tensile mean
+ * stress to zero and holds the void ratio (idep.z) in [e_d, e_i] at the new
+ * pressure. */
 typedef struct lq_vw_params {
     float phi_c, hs, n, ed0, ec0, ei0, alpha, beta;
 } lq_vw_params;
