@@ -926,7 +926,11 @@ extern "C" lq_status lq_engine_neighbour_counts(lq_engine* e, int32_t run_id, in
     LQ_HIP_TRY(hipStreamSynchronize(s));
     sort_resident(e);
     int32_t* nbr = nullptr;
-    LQ_HIP_TRY(hipMalloc(&nbr, (size_t)std::max<int64_t>(e->np_total, 1) * 4));
+    if (hipMalloc(&nbr, (size_t)std::max<int64_t>(e->np_total, 1) * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("lq_engine_neighbour_counts: device allocation failed");
+        return LQ_ERR_OUT_OF_MEMORY;
+    }
     const IndexMap mats{b.mat_idx, 0};
     const int var = e->variant >= 0 ? e->variant : g_variant;
     launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,

@@ -219,6 +219,42 @@ def test_c1_slab_full_run_matches_oracle(gpu):
     assert max(diffs) <= 0.01 * h0, max(diffs)
 
 
+GOLDEN_C2 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_column_trajectory.json")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(GOLDEN_C2), reason="oracle full-run fixture not generated")
+def test_c2_column_full_run_matches_oracle(gpu):
+    """The benchmark's 1.55 M-particle 3D column, all 5,000 steps on the GPU
+    (one neighbour search per step, z-binned cells, autotuned-default
+    variant): runout, centre of mass, height and free-surface profile within
+    1 % of the FP64 oracle's full run (tests/golden/c2_column_trajectory.json,
+    written by tests/golden/make_c2_trajectory.py)."""
+    import json
+
+    from golden.metrics import collapse_metrics
+
+    gold = json.load(open(GOLDEN_C2))
+    case = cases.column_3d(nsteps=gold["steps"])
+    e, (rid,) = _engine([case])
+    e.step(gold["steps"])
+    assert e.info(rid)["steps"] == gold["steps"]
+    got = collapse_metrics(case, e.download(rid)["posd"])
+    e.close()
+    h0 = case.meta["col"][2]
+    print("\nC2 full run, GPU vs oracle:", {k: (round(got[k], 5), round(gold[k], 5))
+                                          for k in ("runout", "com_x", "com_z", "height")})
+    assert got["below_floor"] == 0 and got["count"] == gold["count"]
+    assert abs(got["runout"] - gold["runout"]) <= 0.01 * gold["runout"], (got["runout"], gold["runout"])
+    assert abs(got["com_x"] - gold["com_x"]) <= 0.01 * gold["com_x"], (got["com_x"], gold["com_x"])
+    assert abs(got["com_z"] - gold["com_z"]) <= 0.01 * gold["com_z"], (got["com_z"], gold["com_z"])
+    assert abs(got["height"] - gold["height"]) <= 0.01 * h0, (got["height"], gold["height"])
+    n = min(len(got["profile"]), len(gold["profile"]))
+    diffs = [abs(a - b) for a, b in zip(got["profile"][:n], gold["profile"][:n]) if a is not None and b is not None]
+    assert len(diffs) >= 0.8 * n
+    assert max(diffs) <= 0.01 * h0, max(diffs)
+
+
 def _engine_ordered(case, zbits, steps):
     from loquat_amd.device import Engine
     e = Engine(case.np, case.derived().ncell, max_runs=1)
