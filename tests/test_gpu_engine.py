@@ -263,14 +263,17 @@ def _engine_ordered(case, zbits, steps):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("which,zbits", [("small", 0), ("small", 4), ("c2", 4)])
+@pytest.mark.parametrize("which,zbits", [("small", 0), ("small", 4), ("c2", 4), ("slope", 4)])
 def test_engine_neighbour_counts_exact(gpu, which, zbits):
     """Pruned neighbour scans (column skip, and the z-window over z-binned
     cells when zbits > 0) find exactly the oracle's neighbours, particle by
     particle, on the engine's own moved state (c2: the 1.55 M-particle
-    benchmark column)."""
+    benchmark column; slope: the von Wolffersdorff embankment at 0.3 M
+    particles, an inclined free surface)."""
     if which == "small":
         case = cases.perturb(cases.small_column_3d(seed=61), seed=62)
+    elif which == "slope":
+        case = cases.perturb(cases.slope_vw(npm_target=2e5, nsteps=10), seed=64, v_amp=0.3)
     else:
         case = cases.perturb(cases.column_3d(nsteps=10), seed=63, v_amp=0.3)
     e, rid = _engine_ordered(case, zbits, 4)
