@@ -324,8 +324,9 @@ def main() -> None:
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=None, help="interaction kernel variant (without --autotune)")
-    ap.add_argument("--autotune", type=int, default=1, help="1: time every interaction variant on the "
-                    "workload's state and run the fastest")
+    ap.add_argument("--autotune", type=int, default=1, help="1: time the in-cell orders (from --cell-order, "
+                    "kept unless another is 2 %% faster) and every interaction variant on the workload's state "
+                    "and run the fastest")
     ap.add_argument("--ensemble-runs", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--serve-requests", type=int, default=48)
@@ -364,10 +365,15 @@ def main() -> None:
     eng.set_cell_order(a.cell_order)
     rid = eng.add(case)
     tuned = None
+    cell_order = a.cell_order
     if a.autotune:
+        cell_order, oms = eng.autotune_cell_order(3)
         best, ms = eng.autotune(3)
-        tuned = {"variant": best, "kernel": KERNEL_NAMES[best], "launch_ms": [round(m, 4) for m in ms]}
-        log(f"autotune: variant {best}, launch ms per variant {tuned['launch_ms']}")
+        tuned = {"variant": best, "kernel": KERNEL_NAMES[best], "launch_ms": [round(m, 4) for m in ms],
+                 "cell_order_zbits": cell_order,
+                 "ms_per_cell_order": [round(m, 4) for m in oms]}
+        log(f"autotune: cell order {cell_order} (sort + Adami + interaction ms per z-bin bits 0..: "
+            f"{tuned['ms_per_cell_order']}), variant {best}, launch ms per variant {tuned['launch_ms']}")
     variant = tuned["variant"] if tuned else a.variant
     eng.step(a.warmup)
     eng.sync()
@@ -426,7 +432,7 @@ def main() -> None:
                         "phi=30 deg, Shepard every 20 steps, FP32 sums + FP64 positions",
             "npm": case.npm, "np": case.np, "ncell": p.ncell, "dx": 0.005,
             "particle_step": "one material particle advanced one full predictor-corrector step",
-            "cell_order_zbits": a.cell_order,
+            "cell_order_zbits": cell_order,
             "np_steps_per_s_M": round(case.np * a.steps / (gpu_ms * 1e-3) / 1e6, 3),
             "wall_ms_per_step": round(wall * 1e3 / a.steps, 4),
             "steps_done": info["steps"],

@@ -384,6 +384,18 @@ lq_status lq_engine_profile(lq_engine* eng, int32_t nsteps, double* ms_stage5);
 #define LQ_DEFAULT_INTERACTION_VARIANT 6
 lq_status lq_engine_autotune(lq_engine* eng, int32_t nrep, int32_t* best, double* ms_per_variant);
 
+/* Picks the in-cell order (lq_engine_set_cell_order: 2^b height bins per
+ * cell, b = 0 .. LQ_CELL_ORDER_CANDIDATES - 1; the 2h cell itself is fixed
+ * by the reference's cell hash) by timing, on the resident state, what the
+ * order changes: the sort and permute, Adami and the interaction with the
+ * engine's variant.  ms_per_order (nullable, LQ_CELL_ORDER_CANDIDATES
+ * entries) receives the mean time of those four per candidate.  Leaves the
+ * engine in the fastest order (the order it had, unless another is more than
+ * 2 % faster) with the resident runs sorted by it (results change only in
+ * the order of FP32 sums).  No reference counterpart. */
+#define LQ_CELL_ORDER_CANDIDATES 6
+lq_status lq_engine_autotune_cell_order(lq_engine* eng, int32_t nrep, int32_t* best_bits, double* ms_per_order);
+
 /* Interaction kernel variant for the engine and lq_particle_interaction:
  * 0 = per-thread direct loop, 1 = LDS-compacted neighbour lists,
  * 2 = LDS lists with a 4-wide unrolled candidate scan,

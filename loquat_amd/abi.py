@@ -87,6 +87,7 @@ class LqServerConfig(C.Structure):
 
 REQ_STATE = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "rejected"}
 INTERACTION_VARIANTS = 7  # LQ_INTERACTION_VARIANTS
+CELL_ORDER_CANDIDATES = 6  # LQ_CELL_ORDER_CANDIDATES
 DEFAULT_INTERACTION_VARIANT = 6  # LQ_DEFAULT_INTERACTION_VARIANT
 
 
@@ -146,6 +147,7 @@ _PROTOS = {
     "lq_engine_poll": (i32, [vp, C.POINTER(i32)]),
     "lq_engine_compact": (i32, [vp, C.POINTER(C.c_int64)]),
     "lq_engine_autotune": (i32, [vp, i32, C.POINTER(i32), C.POINTER(f64)]),
+    "lq_engine_autotune_cell_order": (i32, [vp, i32, C.POINTER(i32), C.POINTER(f64)]),
     "lq_engine_run_failed": (i32, [vp, i32, C.POINTER(i32)]),
     "lq_server_create": (i32, [C.POINTER(LqServerConfig), C.POINTER(vp)]),
     "lq_server_destroy": (i32, [vp]),

@@ -1003,6 +1003,75 @@ extern "C" lq_status lq_engine_autotune(lq_engine* e, int32_t nrep, int32_t* bes
     return LQ_OK;
 }
 
+extern "C" lq_status lq_engine_autotune_cell_order(lq_engine* e, int32_t nrep, int32_t* best_bits,
+                                                   double* ms_per_order) {
+    if (!e || nrep <= 0) {
+        set_error("lq_engine_autotune_cell_order: invalid argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (e->runs.empty() || e->nmat_total == 0) {
+        set_error("lq_engine_autotune_cell_order: no material particles resident");
+        return LQ_ERR_STATE;
+    }
+    Buffers& b = e->b;
+    hipStream_t s = e->stream;
+    LQ_HIP_TRY(hipStreamSynchronize(s));
+    const IndexMap walls{b.wall_idx, 0}, mats{b.mat_idx, 0};
+    const int var = e->variant >= 0 ? e->variant : g_variant;
+    // what the in-cell order changes: the sort (key width), Adami and the
+    // interaction (z-window pruning); both scans on a fresh table
+    auto once = [&]() {
+        sort_resident(e);
+        launch_adami_table(b.desc, b.run_of, b.S[e->cur], b.grid.cell_beg, b.grid.cell_end, walls,
+                           (int)e->nwall_total, var, nullptr, s);
+        launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
+                                 (int)e->nmat_total, (int)e->nmat_art, nullptr, var, nullptr, s);
+    };
+    hipEvent_t ev[2];
+    LQ_HIP_TRY(hipEventCreate(&ev[0]));
+    LQ_HIP_TRY(hipEventCreate(&ev[1]));
+    const int start = e->zbits;
+    double ms_all[LQ_CELL_ORDER_CANDIDATES];
+    double best_ms = 1e30;
+    int best_b = start;
+    lq_status st = LQ_OK;
+    for (int bits = 0; bits < LQ_CELL_ORDER_CANDIDATES && st == LQ_OK; ++bits) {
+        st = lq_engine_set_cell_order(e, bits);
+        if (st != LQ_OK) break;
+        once();  // warm-up, and the resident state in this order
+        hipError_t err = hipEventRecord(ev[0], s);
+        for (int r = 0; r < nrep && err == hipSuccess; ++r) once();
+        if (err == hipSuccess) err = hipEventRecord(ev[1], s);
+        if (err == hipSuccess) err = hipEventSynchronize(ev[1]);
+        float ms = 0.f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        if (err != hipSuccess) {
+            set_error(std::string("lq_engine_autotune_cell_order: ") + hipGetErrorString(err));
+            st = LQ_ERR_HIP;
+            break;
+        }
+        const double per = ms / nrep;
+        ms_all[bits] = per;
+        if (ms_per_order) ms_per_order[bits] = per;
+        if (per < best_ms) {
+            best_ms = per;
+            best_b = bits;
+        }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    LQ_TRY(st);
+    LQ_TRY(last_launch("lq_engine_autotune_cell_order"));
+    // the order the engine had stays unless another is clearly faster
+    // (2 %), so that timing noise does not flip it between calls
+    if (start >= 0 && start < LQ_CELL_ORDER_CANDIDATES && ms_all[start] <= 1.02 * best_ms) best_b = start;
+    LQ_TRY(lq_engine_set_cell_order(e, best_b));
+    sort_resident(e);
+    LQ_HIP_TRY(hipStreamSynchronize(s));
+    if (best_bits) *best_bits = best_b;
+    return LQ_OK;
+}
+
 extern "C" lq_status lq_engine_profile(lq_engine* e, int32_t nsteps, double* ms_stage5) {
     if (!e || nsteps <= 0 || !ms_stage5) {
         set_error("lq_engine_profile: invalid argument");

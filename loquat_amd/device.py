@@ -278,6 +278,14 @@ class Engine:
         abi.check(abi.lib().lq_engine_autotune(self.h, int(nrep), C.byref(best), ms))
         return best.value, list(ms)
 
+    def autotune_cell_order(self, nrep: int = 3) -> tuple[int, list[float]]:
+        """Pins the fastest in-cell order (2^b height bins per cell) for this
+        engine, timing sort, permute, Adami and interaction per candidate b."""
+        best = C.c_int32()
+        ms = (C.c_double * abi.CELL_ORDER_CANDIDATES)()
+        abi.check(abi.lib().lq_engine_autotune_cell_order(self.h, int(nrep), C.byref(best), ms))
+        return best.value, list(ms)
+
     def profile(self, nsteps: int) -> list[float]:
         ms = (C.c_double * 5)()
         abi.check(abi.lib().lq_engine_profile(self.h, int(nsteps), ms))

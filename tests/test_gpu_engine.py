@@ -535,3 +535,33 @@ def test_slope_von_wolffersdorff_stays_stable(gpu):
     assert np.abs(vel[:, :3]).max() < 5.0
     assert (s[:, :3].sum(1) <= 0).all()
     assert ev.min() >= case.vw.ed0 * (1 - 1e-6) and ev.max() <= case.vw.ei0 * (1 + 1e-6)
+
+
+@pytest.mark.gpu
+def test_autotune_cell_order_keeps_neighbour_sets(gpu):
+    """The cell-order tuner returns one of its candidates with a positive time
+    for each, leaves the engine in that order, and the neighbour sets after it
+    are still exactly the oracle's; the run then steps normally."""
+    from loquat_amd import abi
+    case = cases.perturb(cases.small_column_3d(seed=81), seed=82)
+    e, rid = _engine_ordered(case, 4, 6)
+    bits, ms = e.autotune_cell_order(2)
+    assert 0 <= bits < abi.CELL_ORDER_CANDIDATES and len(ms) == abi.CELL_ORDER_CANDIDATES
+    assert all(m > 0 for m in ms)
+    # the fastest, or the starting order (4) when within 2 % of the fastest
+    assert bits == int(np.argmin(ms)) or (bits == 4 and ms[4] <= 1.02 * min(ms))
+    got = e.neighbour_counts(rid)
+    h = e.download(rid)
+    p = case.derived()
+    st = O.OState.from_arrays(h["posd"], h["pospres"], h["velrhop"], h["idep"], h["str"], h["stn"])
+    O.initial_sort(p, st)
+    g = O.grid_build(p, st)
+    _, nbr = O.interaction(p, st, g["cell_beg"], g["cell_end"])
+    ref = np.zeros_like(nbr)
+    ref[st.orig] = nbr
+    assert np.array_equal(got[p.npb:], ref[p.npb:])
+    e.step(6)
+    e.sync()
+    assert e.info(rid)["steps"] == 6 and e.info(rid)["finished"]
+    assert np.isfinite(e.download(rid)["velrhop"]).all()
+    e.close()
