@@ -1060,8 +1060,11 @@ extern "C" lq_status lq_engine_autotune_cell_order(lq_engine* e, int32_t nrep, i
     }
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
-    LQ_TRY(st);
-    LQ_TRY(last_launch("lq_engine_autotune_cell_order"));
+    if (st == LQ_OK) st = last_launch("lq_engine_autotune_cell_order");
+    if (st != LQ_OK) {
+        (void)lq_engine_set_cell_order(e, start);  // leave the engine as it was
+        return st;
+    }
     // the order the engine had stays unless another is clearly faster
     // (2 %), so that timing noise does not flip it between calls
     if (start >= 0 && start < LQ_CELL_ORDER_CANDIDATES && ms_all[start] <= 1.02 * best_ms) best_b = start;
