@@ -513,10 +513,10 @@ def test_run_moving_beyond_the_table_reach_is_stopped(gpu):
 @pytest.mark.gpu
 def test_slope_von_wolffersdorff_stays_stable(gpu):
     """The von Wolffersdorff embankment at 0.3 M particles runs 3,000 steps
-    (0.105 s, three times past where the free-surface tension used to set
-    off a blow-up at the toe) without a failure flag: finite state, void
-    ratio inside [e_d0, e_i0], no tensile mean stress, particles slower than
-    5 m/s."""
+    (0.105 s; without the model's tension cut and void-ratio bounds this
+    embankment blows up at its toe near 0.035 s) without a failure flag:
+    finite state, void ratio inside [e_d0, e_i0], no tensile mean stress,
+    particles slower than 5 m/s."""
     import ctypes as C
     from loquat_amd import abi
     case = cases.slope_vw(npm_target=2e5, nsteps=3000)
