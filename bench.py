@@ -333,6 +333,8 @@ def main() -> None:
     ap.add_argument("--serve-step-scale", type=float, default=0.05)
     ap.add_argument("--slope-steps", type=int, default=20)
     ap.add_argument("--no-extra", action="store_true", help="skip the ensemble / served / slope sections")
+    ap.add_argument("--slope-only", action="store_true",
+                    help="time only the embankment section and print its JSON (for profiling that workload)")
     ap.add_argument("--cell-order", type=int, default=4, metavar="B",
                     help="in-cell order of the step's sort: 0 = by cell only, B > 0 = 2^B z-bins per cell")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
@@ -350,6 +352,9 @@ def main() -> None:
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if a.slope_only:
+        print(json.dumps({"slope_vw": slope_section(a.slope_steps)}), flush=True)
+        return
     if a.variant is None:
         a.variant = abi.DEFAULT_INTERACTION_VARIANT
     abi.check(abi.lib().lq_set_interaction_variant(a.variant))
