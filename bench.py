@@ -284,9 +284,11 @@ def served_section(nreq: int, step_scale: float, seed: int = 0) -> dict:
     }
 
 
-def slope_section(steps: int, npm_target: float = 4.0e6) -> dict:
+def slope_section(steps: int, npm_target: float = 4.0e6, autotune: bool = True, cell_order: int = 4) -> dict:
     """30-degree embankment, ~4 M material particles, von Wolffersdorff with a
-    seeded +-5% void-ratio field."""
+    seeded +-5% void-ratio field.  With autotune, the in-cell order (starting
+    from cell_order) and then the interaction variant are tuned on the run's
+    state before the 5 untimed steps."""
     import torch
 
     from loquat_amd import cases
@@ -294,7 +296,13 @@ def slope_section(steps: int, npm_target: float = 4.0e6) -> dict:
 
     c = cases.slope_vw(npm_target=npm_target, nsteps=steps + 5)
     eng = Engine(c.np, c.derived().ncell, 1)
+    eng.set_cell_order(cell_order)
     rid = eng.add(c)
+    tuned = None
+    if autotune:
+        bits, _ = eng.autotune_cell_order(3)
+        var, _ = eng.autotune(3)
+        tuned = {"cell_order_zbits": bits, "variant": var}
     eng.step(5)
     eng.sync()
     s = torch.cuda.ExternalStream(eng.stream())
@@ -312,7 +320,7 @@ def slope_section(steps: int, npm_target: float = 4.0e6) -> dict:
     return {"workload": f"30-degree embankment 1.0 x 3.0 x 0.5 m, npm={c.npm}, np={c.np}, "
                         f"dx={c.meta['dx'] * 1e3:.2f} mm, von Wolffersdorff",
             "M_particle_steps_s": round(c.npm * steps / (ms * 1e-3) / 1e6, 3),
-            "ms_per_step": round(ms / steps, 4), "steps": steps}
+            "ms_per_step": round(ms / steps, 4), "steps": steps, "autotune": tuned}
 
 
 def main() -> None:
@@ -353,7 +361,8 @@ def main() -> None:
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     if a.slope_only:
-        print(json.dumps({"slope_vw": slope_section(a.slope_steps)}), flush=True)
+        sec = slope_section(a.slope_steps, autotune=bool(a.autotune), cell_order=a.cell_order)
+        print(json.dumps({"slope_vw": sec}), flush=True)
         return
     if a.variant is None:
         a.variant = abi.DEFAULT_INTERACTION_VARIANT
@@ -487,7 +496,7 @@ def main() -> None:
         out["served"] = served_section(a.serve_requests, a.serve_step_scale)
         log(f"served: {out['served']} ({time.perf_counter() - t:.1f} s)")
         t = time.perf_counter()
-        out["slope_vw"] = slope_section(a.slope_steps)
+        out["slope_vw"] = slope_section(a.slope_steps, autotune=bool(a.autotune), cell_order=a.cell_order)
         log(f"slope: {out['slope_vw']} ({time.perf_counter() - t:.1f} s)")
     if not a.no_cpu:
         log(f"CPU baseline: FP64 oracle, {a.cpu_steps} steps ...")
