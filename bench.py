@@ -333,7 +333,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=None, help="interaction kernel variant (without --autotune)")
     ap.add_argument("--autotune", type=int, default=1, help="1: time the in-cell orders (from --cell-order, "
-                    "kept unless another is 2 %% faster) and every interaction variant on the workload's state "
+                    "kept unless another is 1 %% faster) and every interaction variant on the workload's state "
                     "and run the fastest")
     ap.add_argument("--ensemble-runs", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=200)

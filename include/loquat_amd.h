@@ -391,7 +391,7 @@ lq_status lq_engine_autotune(lq_engine* eng, int32_t nrep, int32_t* best, double
  * engine's variant.  ms_per_order (nullable, LQ_CELL_ORDER_CANDIDATES
  * entries) receives the mean time of those four per candidate.  Leaves the
  * engine in the fastest order (the order it had, unless another is more than
- * 2 % faster) with the resident runs sorted by it (results change only in
+ * 1 % faster) with the resident runs sorted by it (results change only in
  * the order of FP32 sums).  No reference counterpart. */
 #define LQ_CELL_ORDER_CANDIDATES 6
 lq_status lq_engine_autotune_cell_order(lq_engine* eng, int32_t nrep, int32_t* best_bits, double* ms_per_order);

@@ -1066,8 +1066,9 @@ extern "C" lq_status lq_engine_autotune_cell_order(lq_engine* e, int32_t nrep, i
         return st;
     }
     // the order the engine had stays unless another is clearly faster
-    // (2 %), so that timing noise does not flip it between calls
-    if (start >= 0 && start < LQ_CELL_ORDER_CANDIDATES && ms_all[start] <= 1.02 * best_ms) best_b = start;
+    // (1 %, above the timing noise), so that calls do not flip between
+    // near-equal orders
+    if (start >= 0 && start < LQ_CELL_ORDER_CANDIDATES && ms_all[start] <= 1.01 * best_ms) best_b = start;
     LQ_TRY(lq_engine_set_cell_order(e, best_b));
     sort_resident(e);
     LQ_HIP_TRY(hipStreamSynchronize(s));
