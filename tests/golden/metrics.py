@@ -31,3 +31,12 @@ def collapse_metrics(case, posd) -> dict:
     return {"runout": float(np.percentile(x[base], 99)), "com_x": float(x.mean()), "com_z": float(z.mean()),
             "height": float(np.percentile(z, 99)), "bin": w, "profile": prof,
             "below_floor": int(np.sum(z < 0)), "count": int(x.size)}
+
+
+def displacement_metrics(case, posd) -> dict:
+    """Mean and 99th percentile of the material particles' displacement from
+    the case's initial positions (m), over the particles still finite."""
+    npb = case.par.npb
+    d = np.linalg.norm(np.asarray(posd[npb:], np.float64) - case.posd[npb:], axis=1)
+    d = d[np.isfinite(d)]
+    return {"disp_mean": float(d.mean()), "disp_p99": float(np.percentile(d, 99))}

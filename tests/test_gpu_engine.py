@@ -255,6 +255,44 @@ def test_c2_column_full_run_matches_oracle(gpu):
     assert max(diffs) <= 0.01 * h0, max(diffs)
 
 
+GOLDEN_C3 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c3_slope_trajectory.json")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(GOLDEN_C3), reason="oracle full-run fixture not generated")
+def test_c3_slope_full_run_matches_oracle(gpu):
+    """The von Wolffersdorff embankment at reduced resolution (323,843
+    particles), 3,000 steps on the GPU: runout, centre of mass, height and
+    free-surface profile within 1 % of the FP64 oracle's run, and the
+    displacement field's mean and 99th percentile within 1 % of the oracle's
+    (tests/golden/c3_slope_trajectory.json, written by
+    tests/golden/make_c3_trajectory.py)."""
+    import json
+
+    from golden.metrics import collapse_metrics, displacement_metrics
+
+    gold = json.load(open(GOLDEN_C3))
+    case = cases.slope_vw(npm_target=gold["npm_target"], nsteps=gold["steps"])
+    e, (rid,) = _engine([case])
+    e.step(gold["steps"])
+    assert e.info(rid)["steps"] == gold["steps"]
+    posd = e.download(rid)["posd"]
+    e.close()
+    got = collapse_metrics(case, posd)
+    got.update(displacement_metrics(case, posd))
+    keys = ("runout", "com_x", "com_z", "height", "disp_mean", "disp_p99")
+    print("\nC3 slope full run, GPU vs oracle:", {k: (round(got[k], 6), round(gold[k], 6)) for k in keys})
+    h0 = 1.0   # embankment height, m
+    assert got["below_floor"] == 0 and got["count"] == gold["count"]
+    for k in ("runout", "com_x", "com_z", "disp_mean", "disp_p99"):
+        assert abs(got[k] - gold[k]) <= 0.01 * abs(gold[k]), (k, got[k], gold[k])
+    assert abs(got["height"] - gold["height"]) <= 0.01 * h0, (got["height"], gold["height"])
+    n = min(len(got["profile"]), len(gold["profile"]))
+    diffs = [abs(a - b) for a, b in zip(got["profile"][:n], gold["profile"][:n]) if a is not None and b is not None]
+    assert len(diffs) >= 0.8 * n
+    assert max(diffs) <= 0.01 * h0, max(diffs)
+
+
 def _engine_ordered(case, zbits, steps):
     from loquat_amd.device import Engine
     e = Engine(case.np, case.derived().ncell, max_runs=1)
