@@ -214,21 +214,7 @@ def served_section(nreq: int, step_scale: float, seed: int = 0) -> dict:
     from loquat_amd.device import Engine
     from loquat_amd.serve import Server
 
-    rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / 20.0, nreq))
-    reqs = []
-    for i in range(nreq):
-        npm = float(np.clip(math.exp(rng.normal(math.log(3e4), 1.0)), 5e3, 5e5))
-        steps = max(1, int(round(rng.integers(2000, 8001) * step_scale)))
-        if rng.random() < 0.5:
-            aspect = rng.uniform(0.5, 4.0)
-            nw = max(2, round((npm / aspect) ** (1 / 3)))
-            w, hh = nw * 0.005, max(2, round(aspect * nw)) * 0.005
-            c = cases.column(name=f"req{i}", dx=0.005, col=(w, w, hh), box=(w + 2 * hh, w), quasi2d=False,
-                             phi=rng.uniform(20, 40), c=rng.uniform(0, 5e3), shepard=20, nsteps=steps, seed=i)
-        else:
-            c = cases.slope_vw(npm_target=npm, nsteps=steps, seed=i, depth=0.1)
-        reqs.append((arrivals[i], c, steps))
+    reqs = cases.served_mix(nreq, step_scale, seed)
     np_cap = sum(c.np for _, c, _ in reqs)
     nc_cap = sum(c.derived().ncell for _, c, _ in reqs)
     srv = Server(np_cap, nc_cap, 256, 288 * 10**9, steps_per_tick=25)

@@ -42,6 +42,14 @@ typedef enum lq_status {
     LQ_ERR_STATE = 10         /* call made in the wrong order (e.g. permute before build) */
 } lq_status;
 
+/* Largest particle count of any call, grid or engine.  Neighbour gathers use
+ * 32-bit byte offsets from each array's base (one shift per load instead of
+ * 64-bit address arithmetic); the widest gathered element is a 24-byte
+ * tensor2d, so indices must stay below 2^32 / 24.  Larger capacities return
+ * LQ_ERR_CAPACITY before anything is allocated (about 97 GB of engine state;
+ * a larger ensemble is split across engines, which share nothing). */
+#define LQ_MAX_PARTICLES 178956970
+
 /* hipStream_t passed opaquely; NULL is the null stream. */
 typedef void* lq_stream;
 
@@ -289,7 +297,8 @@ typedef struct lq_run_input {
  * with nsteps <= 0 ends on the device once time >= time_max;
  * lq_engine_run_info reports the device's step count and time. */
 
-/* Capacities bound the packed layout: particles, cells (sum over runs), runs. */
+/* Capacities bound the packed layout: particles (at most LQ_MAX_PARTICLES),
+ * cells (sum over runs), runs. */
 lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t max_runs, lq_engine** out);
 lq_status lq_engine_destroy(lq_engine* eng);
 
@@ -396,7 +405,12 @@ lq_status lq_engine_autotune(lq_engine* eng, int32_t nrep, int32_t* best, double
 #define LQ_CELL_ORDER_CANDIDATES 6
 lq_status lq_engine_autotune_cell_order(lq_engine* eng, int32_t nrep, int32_t* best_bits, double* ms_per_order);
 
-/* Interaction kernel variant for the engine and lq_particle_interaction:
+/* Interaction kernel variant.  lq_set_interaction_variant sets it for the
+ * module-level calls (lq_particle_interaction, lq_adami_boundary, ...) made by
+ * the calling thread, and as the initial variant of engines that thread
+ * creates afterwards; each engine then keeps its own
+ * (lq_engine_set_interaction_variant, lq_engine_autotune), so handles on
+ * different threads or streams do not share it.
  * 0 = per-thread direct loop, 1 = LDS-compacted neighbour lists,
  * 2 = LDS lists with a 4-wide unrolled candidate scan,
  * 3 = variant 2 with offset loads, pruned column scans, XCD-contiguous
@@ -408,6 +422,8 @@ lq_status lq_engine_autotune_cell_order(lq_engine* eng, int32_t nrep, int32_t* b
  * Variants 3 and up also select the list-based Adami and Shepard scans. */
 lq_status lq_set_interaction_variant(int32_t variant);
 int32_t lq_get_interaction_variant(void);
+lq_status lq_engine_set_interaction_variant(lq_engine* eng, int32_t variant);
+int32_t lq_engine_get_interaction_variant(const lq_engine* eng);
 
 /* ----------------------------------------------------------------- server
  * Continuous batching of independent runs (no counterpart in the reference,
@@ -434,16 +450,23 @@ typedef enum lq_request_state {
     LQ_REQ_RUNNING = 1,
     LQ_REQ_DONE = 2,
     LQ_REQ_FAILED = 3,        /* non-finite state: evicted, others unaffected */
-    LQ_REQ_REJECTED = 4       /* larger than the budget or the capacities */
+    LQ_REQ_REJECTED = 4       /* larger than the budget or the capacities, or failing the
+                                 engine's checks (lq_engine_add_run's errors) */
 } lq_request_state;
 
 lq_status lq_server_create(const lq_server_config* cfg, lq_server** out);
 lq_status lq_server_destroy(lq_server* srv);
-/* Queues a run (host arrays are copied).  request_id is set even when the
- * request is rejected (state LQ_REQ_REJECTED). */
+/* Queues a run (host arrays are copied).  The checks lq_engine_add_run makes
+ * (derived parameters, constitutive model and its constants, walls inside the
+ * grid) are made here, so an invalid request is rejected at once instead of
+ * blocking the queue.  request_id is set even when the request is rejected
+ * (state LQ_REQ_REJECTED). */
 lq_status lq_server_submit(lq_server* srv, const lq_run_input* in, int32_t* request_id);
 /* One admission / step / eviction round.  finished_ids receives up to cap ids
- * of requests that completed or failed in this tick; nfinished the count. */
+ * of requests that completed, failed or were rejected at admission in this
+ * tick; nfinished the count.  A request the engine refuses at admission is
+ * rejected and dropped from the queue (the requests behind it go on); a
+ * device error there is returned after the request is dropped. */
 lq_status lq_server_tick(lq_server* srv, int32_t* finished_ids, int32_t cap, int32_t* nfinished);
 lq_status lq_server_request_info(const lq_server* srv, int32_t request_id, int32_t* state, int64_t* steps_done,
                                  int64_t* nsteps, int64_t* bytes);

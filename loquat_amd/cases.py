@@ -12,6 +12,7 @@ Configurations:
   column_3d         3D column 80 x 80 x 160 at 5 mm (1,024,000 particles)
   slope_vw          30-degree embankment, von Wolffersdorff, ~4 M particles
   ensemble          256 columns of log-uniform 2k..60k particles
+  served_mix        Poisson request stream of columns and embankments, 5k..500k
 """
 from __future__ import annotations
 
@@ -265,6 +266,35 @@ def ensemble(nruns: int = 256, seed: int = 0, dx: float = 0.005, nsteps: int = 5
         case = column(name=f"ens{r}", dx=dx, col=(w, w, hgt), box=(w + 2.0 * hgt, w), quasi2d=False,
                       phi=phi, c=c, shepard=20, nsteps=nsteps, seed=seed * 1000 + r)
         out.append(case)
+    return out
+
+
+def served_mix(nreq: int = 48, step_scale: float = 1.0, seed: int = 0, rate: float = 20.0,
+               extremes: bool = False) -> list[tuple[float, Case, int]]:
+    """Request stream of the served workload: Poisson arrivals (rate per
+    second), column (Drucker-Prager, phi U[20,40] deg, c U[0,5] kPa, aspect
+    U[0.5,4]) or 0.1 m-deep embankment (von Wolffersdorff) 50/50, material
+    particles log-normal around 30k clipped to [5k, 500k], 2,000-8,000 steps
+    scaled by step_scale.  extremes: the first two requests take the clip
+    bounds (5k and 500k particles) so that a short stream covers both ends.
+    Returns (arrival time s, case, steps) in arrival order."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    out = []
+    for i in range(nreq):
+        npm = float(np.clip(math.exp(rng.normal(math.log(3e4), 1.0)), 5e3, 5e5))
+        if extremes and i < 2:
+            npm = (5e3, 5e5)[i]
+        steps = max(1, int(round(rng.integers(2000, 8001) * step_scale)))
+        if rng.random() < 0.5:
+            aspect = rng.uniform(0.5, 4.0)
+            nw = max(2, round((npm / aspect) ** (1 / 3)))
+            w, hh = nw * 0.005, max(2, round(aspect * nw)) * 0.005
+            c = column(name=f"req{i}", dx=0.005, col=(w, w, hh), box=(w + 2 * hh, w), quasi2d=False,
+                       phi=rng.uniform(20, 40), c=rng.uniform(0, 5e3), shepard=20, nsteps=steps, seed=i)
+        else:
+            c = slope_vw(npm_target=npm, nsteps=steps, seed=i, depth=0.1)
+        out.append((float(arrivals[i]), c, steps))
     return out
 
 

@@ -3,11 +3,15 @@
 // runs packed in one segmented layout.
 //
 // Engine step (src/loquat.cu:322-409), every stage one launch over all runs:
-//   step_begin                       per-run Shepard flag (src/loquat.cu:400)
-//   [predict] grid build, permute Primary (src/loquat.cu:345-347),
-//             Adami, interaction, predictor (+ fused *_ini copy)
-//   [correct] grid build, permute All (src/loquat.cu:373-375),
-//             Adami, interaction, corrector
+//   step_begin                       per-run Shepard flag (src/loquat.cu:400),
+//                                    dynamic time step when a run asks for it
+//   [predict] grid build (hash, sort, cell table) and permute Primary
+//             (src/loquat.cu:345-347), Adami, interaction, predictor (+ the
+//             fused *_ini copy and each run's largest displacement)
+//   [correct] no second neighbour search: Adami and the interaction scan the
+//             predictor half's cell table with their reach widened by the
+//             run's largest displacement (the neighbour sets of the reference's
+//             second search, src/loquat.cu:362-375), then the corrector
 //   [shepard] if any run is due: grid build, permute Primary with the stress
 //             routed to scratch, Shepard filter back into the state
 //   step_end                         per-run time += dt, step++, finish flag
@@ -26,7 +30,9 @@ namespace lq {
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 
-static int g_variant = LQ_DEFAULT_INTERACTION_VARIANT;
+// Interaction variant of module-level calls made by this thread, and the
+// initial variant of engines this thread creates (each engine keeps its own).
+static thread_local int g_variant = LQ_DEFAULT_INTERACTION_VARIANT;
 
 // Smallest FP32 value T with sqrtf(T) * i_h >= 2 (both correctly rounded):
 // r2 < T is then exactly the predicate q = sqrtf(r2) * i_h < 2 for every
@@ -88,6 +94,10 @@ static lq_status check_params(const lq_params* par) {
     if (par->np < 0 || par->npb < 0 || par->npm < 0 || par->np != par->npb + par->npm) {
         set_error("invalid particle counts: np must equal npb + npm");
         return LQ_ERR_INVALID_ARG;
+    }
+    if (par->np > LQ_MAX_PARTICLES) {
+        set_error("more than LQ_MAX_PARTICLES particles (32-bit gather offsets)");
+        return LQ_ERR_CAPACITY;
     }
     if (par->ncell <= 0 || par->ncell != 2 * par->ncell_one_layer + 1 ||
         (int64_t)par->cell_num_x * par->cell_num_y * par->cell_num_z != par->ncell_one_layer) {
@@ -161,9 +171,13 @@ struct lq_grid {
 };
 
 extern "C" lq_status lq_grid_create(int64_t np_cap, int64_t ncell_cap, lq_grid** out) {
-    if (!out || np_cap < 0 || ncell_cap < 0 || np_cap > 0x7fffffff || ncell_cap > 0x7fffffff) {
+    if (!out || np_cap < 0 || ncell_cap < 0 || ncell_cap > 0x7fffffff) {
         set_error("lq_grid_create: invalid capacity");
         return LQ_ERR_INVALID_ARG;
+    }
+    if (np_cap > LQ_MAX_PARTICLES) {
+        set_error("lq_grid_create: np_cap exceeds LQ_MAX_PARTICLES (32-bit gather offsets)");
+        return LQ_ERR_CAPACITY;
     }
     lq_grid* g = new lq_grid();
     lq_status st = alloc_grid(g->g, np_cap, ncell_cap);
@@ -397,7 +411,8 @@ struct lq_engine {
     int64_t nmat_total = 0, nwall_total = 0;
     int32_t next_id = 0;
     bool use_graphs = true;
-    int variant = -1;  // interaction variant pinned by lq_engine_autotune (-1: library default)
+    int variant = LQ_DEFAULT_INTERACTION_VARIANT;  // interaction variant (set at creation, by the autotuner or
+                                                   // lq_engine_set_interaction_variant)
     int zbits = 4;     // z-bins per cell (log2) in the engine's sort key
     bool any_dynamic = false;  // some resident run has dynamic-time-step = 1
     int64_t nmat_art = 0;      // material particles of runs with artificial-pressure = 1 (last in mat_idx)
@@ -464,7 +479,7 @@ int enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
     hipStream_t s = e->stream;
     const int n = (int)e->np_total;
     const int nr = (int)e->runs.size();
-    const int var = e->variant >= 0 ? e->variant : g_variant;
+    const int var = e->variant;
     launch_step_begin(b.desc, nr, b.dmax, b.nout, s);
     if (e->any_dynamic)
         launch_dynamic_dt(b.desc, b.run_of, b.S[cur], b.R, IndexMap{b.mat_idx, 0}, (int)e->nmat_total, nr, b.dtred, s);
@@ -548,6 +563,8 @@ lq_status sync_descs(lq_engine* e) {
     std::vector<RunDesc> h(nr);
     LQ_HIP_TRY(hipStreamSynchronize(e->stream));
     LQ_HIP_TRY(hipMemcpy(h.data(), e->b.desc, nr * sizeof(RunDesc), hipMemcpyDeviceToHost));
+    std::vector<int32_t> f(nr, 0);
+    LQ_HIP_TRY(hipMemcpy(f.data(), e->b.fail, nr * 4, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < nr; ++i) {
         RunDesc& d = e->runs[i].desc;
         d.step = h[i].step;
@@ -555,6 +572,12 @@ lq_status sync_descs(lq_engine* e) {
         d.par.dt = h[i].par.dt;
         d.active = h[i].active;
         d.shepard_due = h[i].shepard_due;
+        // the corrector's failure flag, before any upload writes the host
+        // copy back (lq_engine_poll reads it the same way)
+        if (f[i] && !e->runs[i].failed) {
+            e->runs[i].failed = 1;
+            d.active = 0;
+        }
     }
     return LQ_OK;
 }
@@ -646,11 +669,16 @@ extern "C" int64_t lq_engine_bytes_per_run(int64_t np, int64_t ncell) {
 }
 
 extern "C" lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t max_runs, lq_engine** out) {
-    if (!out || np_cap <= 0 || ncell_cap <= 0 || max_runs <= 0 || np_cap > 0x7fffffff || ncell_cap > 0x7fffffff) {
+    if (!out || np_cap <= 0 || ncell_cap <= 0 || max_runs <= 0 || ncell_cap > 0x7fffffff) {
         set_error("lq_engine_create: invalid capacity");
         return LQ_ERR_INVALID_ARG;
     }
+    if (np_cap > LQ_MAX_PARTICLES) {
+        set_error("lq_engine_create: np_cap exceeds LQ_MAX_PARTICLES (32-bit gather offsets)");
+        return LQ_ERR_CAPACITY;
+    }
     lq_engine* e = new lq_engine();
+    e->variant = g_variant;
     e->np_cap = np_cap;
     e->ncell_cap = ncell_cap;
     e->max_runs = max_runs;
@@ -715,6 +743,19 @@ extern "C" lq_status lq_engine_set_graphs(lq_engine* e, int32_t use_graphs) {
     return LQ_OK;
 }
 
+extern "C" lq_status lq_engine_set_interaction_variant(lq_engine* e, int32_t variant) {
+    if (!e || variant < 0 || variant >= LQ_INTERACTION_VARIANTS) {
+        set_error("lq_engine_set_interaction_variant: variant must be 0..6");
+        return LQ_ERR_INVALID_ARG;
+    }
+    LQ_HIP_TRY(hipStreamSynchronize(e->stream));
+    e->variant = variant;
+    drop_graphs(e);
+    return LQ_OK;
+}
+
+extern "C" int32_t lq_engine_get_interaction_variant(const lq_engine* e) { return e ? e->variant : -1; }
+
 extern "C" lq_status lq_engine_set_cell_order(lq_engine* e, int32_t zbin_bits) {
     if (!e || zbin_bits < 0 || zbin_bits > 8) {
         set_error("lq_engine_set_cell_order: zbin_bits must be 0..8");
@@ -736,20 +777,49 @@ static int64_t steps_until(double t, double tmax, double dt) {
     return n;
 }
 
-extern "C" lq_status lq_engine_add_run(lq_engine* e, const lq_run_input* in, int32_t* run_id) {
-    if (!e || !in || !run_id || !in->host.pospres || !in->host.velrhop || !in->host.idep || !in->host.str ||
-        !in->host.stn) {
-        set_error("lq_engine_add_run: null argument");
+namespace lq {
+// Everything lq_engine_add_run checks about a run before it touches the
+// device: arguments, derived parameters, model-3 constants, and that every
+// wall particle lies inside the cell grid (the wall/material split is by
+// index after each sort; a wall sent to ncell_delete would land past the
+// material block).  par_out receives the derived parameters.  Host only, so
+// the server validates requests at submission with it.
+lq_status check_run_input(const lq_run_input* in, lq_params* par_out) {
+    if (!in || !in->host.pospres || !in->host.velrhop || !in->host.idep || !in->host.str || !in->host.stn) {
+        set_error("run input: null argument");
         return LQ_ERR_INVALID_ARG;
     }
     lq_params par = in->par;
     LQ_TRY(lq_params_derive(&par));
     LQ_TRY(check_params(&par));
-    LQ_TRY(sync_descs(e));
     if (par.constitutive_model == 3 && !(in->vw.phi_c > 0.0f && in->vw.hs > 0.0f && in->vw.n > 0.0f)) {
         set_error("constitutive model 3 needs phi_c, hs, n > 0");
         return LQ_ERR_INVALID_ARG;
     }
+    for (int i = 0; i < par.npb; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            const float v = in->posd ? (float)in->posd[3 * (size_t)i + a] : in->host.pospres[4 * (size_t)i + a];
+            const int n = a == 0 ? par.cell_num_x : (a == 1 ? par.cell_num_y : par.cell_num_z);
+            const float f = floorf((0.5f * v) * par.i_h);
+            if (!(f >= 0.0f) || !(f < (float)n)) {
+                set_error("wall particle " + std::to_string(i) + " lies outside the cell grid");
+                return LQ_ERR_DOMAIN;
+            }
+        }
+    }
+    *par_out = par;
+    return LQ_OK;
+}
+}  // namespace lq
+
+extern "C" lq_status lq_engine_add_run(lq_engine* e, const lq_run_input* in, int32_t* run_id) {
+    if (!e || !run_id) {
+        set_error("lq_engine_add_run: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    lq_params par;
+    LQ_TRY(check_run_input(in, &par));
+    LQ_TRY(sync_descs(e));
     if ((int64_t)e->runs.size() >= e->max_runs) {
         set_error("engine run table is full");
         return LQ_ERR_CAPACITY;
@@ -774,20 +844,6 @@ extern "C" lq_status lq_engine_add_run(lq_engine* e, const lq_run_input* in, int
             xd[i] = pp[4 * (size_t)i];
             yd[i] = pp[4 * (size_t)i + 1];
             zd[i] = pp[4 * (size_t)i + 2];
-        }
-    }
-    // Walls must stay inside the grid: the wall/material split is by index
-    // after each sort, and a wall sent to ncell_delete would land past the
-    // material block.
-    for (int i = 0; i < par.npb; ++i) {
-        for (int a = 0; a < 3; ++a) {
-            const float v = pospres[4 * (size_t)i + a];
-            const int n = a == 0 ? par.cell_num_x : (a == 1 ? par.cell_num_y : par.cell_num_z);
-            const float f = floorf((0.5f * v) * par.i_h);
-            if (!(f >= 0.0f) || !(f < (float)n)) {
-                set_error("wall particle " + std::to_string(i) + " lies outside the cell grid");
-                return LQ_ERR_DOMAIN;
-            }
         }
     }
     LQ_HIP_TRY(hipStreamSynchronize(e->stream));
@@ -932,7 +988,7 @@ extern "C" lq_status lq_engine_neighbour_counts(lq_engine* e, int32_t run_id, in
         return LQ_ERR_OUT_OF_MEMORY;
     }
     const IndexMap mats{b.mat_idx, 0};
-    const int var = e->variant >= 0 ? e->variant : g_variant;
+    const int var = e->variant;
     launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
                              (int)e->nmat_total, (int)e->nmat_art, nbr, var, nullptr, s);
     const RunDesc& d = e->runs[i].desc;
@@ -999,6 +1055,7 @@ extern "C" lq_status lq_engine_autotune(lq_engine* e, int32_t nrep, int32_t* bes
     (void)hipEventDestroy(ev[1]);
     LQ_TRY(last_launch("lq_engine_autotune"));
     e->variant = best_v;
+    drop_graphs(e);
     if (best) *best = best_v;
     return LQ_OK;
 }
@@ -1017,7 +1074,7 @@ extern "C" lq_status lq_engine_autotune_cell_order(lq_engine* e, int32_t nrep, i
     hipStream_t s = e->stream;
     LQ_HIP_TRY(hipStreamSynchronize(s));
     const IndexMap walls{b.wall_idx, 0}, mats{b.mat_idx, 0};
-    const int var = e->variant >= 0 ? e->variant : g_variant;
+    const int var = e->variant;
     // what the in-cell order changes: the sort (key width), Adami and the
     // interaction (z-window pruning); both scans on a fresh table
     auto once = [&]() {

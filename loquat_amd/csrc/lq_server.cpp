@@ -25,6 +25,7 @@
 
 namespace lq {
 void set_error(const std::string& msg);
+lq_status check_run_input(const lq_run_input* in, lq_params* par_out);
 }
 
 namespace {
@@ -39,6 +40,7 @@ struct Request {
     int64_t bytes = 0;
     int32_t ncell = 0;
     int32_t state = LQ_REQ_QUEUED;
+    std::string why;         // reason of a rejection
     int32_t run = -1;        // engine run id while running
     int64_t steps_done = 0;
     bool bad_input = false;  // fake backend: non-finite input
@@ -98,12 +100,15 @@ extern "C" lq_status lq_server_submit(lq_server* s, const lq_run_input* in, int3
         return LQ_ERR_INVALID_ARG;
     }
     lq_params par = in->par;
-    lq_status st = lq_params_derive(&par);
-    if (st != LQ_OK) return st;
-    if (par.np < 0 || par.np != par.npb + par.npm) {
-        lq::set_error("lq_server_submit: np != npb + npm");
+    if (lq_params_derive(&par) != LQ_OK || par.np < 0 || par.np != par.npb + par.npm) {
+        lq::set_error("lq_server_submit: np != npb + npm or parameters that cannot be derived");
         return LQ_ERR_INVALID_ARG;
     }
+    // the checks the engine makes when the run is admitted, made now: a
+    // request that would fail them is rejected here instead of blocking the
+    // head of the queue
+    lq_params checked;
+    const lq_status valid = lq::check_run_input(in, &checked);
     Request r;
     r.par = in->par;
     r.vw = in->vw;
@@ -131,11 +136,23 @@ extern "C" lq_status lq_server_submit(lq_server* s, const lq_run_input* in, int3
                     finite_all(r.str.data(), r.str.size()));
     r.id = s->next_id++;
     const lq_server_config& c = s->cfg;
-    if (r.bytes > c.hbm_budget || par.np > c.np_cap || par.ncell > c.ncell_cap) {
+    if (valid != LQ_OK) {
         r.state = LQ_REQ_REJECTED;
+        r.why = lq_last_error();
+    } else if (r.bytes > c.hbm_budget || par.np > c.np_cap || par.ncell > c.ncell_cap) {
+        r.state = LQ_REQ_REJECTED;
+        r.why = "larger than the HBM budget or the engine capacities";
     } else {
         r.state = LQ_REQ_QUEUED;
         s->queue.push_back(r.id);
+    }
+    if (r.state == LQ_REQ_REJECTED) {
+        std::vector<float>().swap(r.pospres);
+        std::vector<float>().swap(r.velrhop);
+        std::vector<float>().swap(r.idep);
+        std::vector<float>().swap(r.str);
+        std::vector<float>().swap(r.stn);
+        std::vector<double>().swap(r.posd);
     }
     *request_id = r.id;
     s->reqs[r.id] = std::move(r);
@@ -207,15 +224,33 @@ extern "C" lq_status lq_server_tick(lq_server* s, int32_t* finished_ids, int32_t
         return LQ_ERR_INVALID_ARG;
     }
     const lq_server_config& c = s->cfg;
-    // 1. admission, strict FIFO
+    // 1. admission, strict FIFO.  A request the engine refuses (submission
+    // already ran the engine's checks, so this is rare: a device error) is
+    // rejected and dropped from the queue, so it cannot block the requests
+    // behind it; a device error is still returned once it is dropped.
+    int32_t nout = 0;
+    lq_status admit_err = LQ_OK;
     while (!s->queue.empty()) {
         Request& r = s->reqs[s->queue.front()];
         const bool fits = s->bytes_in_use + r.bytes <= c.hbm_budget && s->np_in_use + r.par.np <= c.np_cap &&
                           s->ncell_in_use + r.ncell <= c.ncell_cap && (int32_t)s->running.size() < c.max_runs;
         if (!fits) break;
         lq_status st = admit(s, r);
-        if (st != LQ_OK) return st;
         s->queue.pop_front();
+        if (st != LQ_OK) {
+            r.state = LQ_REQ_REJECTED;
+            r.why = lq_last_error();
+            if (nout < cap) finished_ids[nout] = r.id;
+            ++nout;
+            if (st == LQ_ERR_HIP || st == LQ_ERR_OUT_OF_MEMORY) {
+                admit_err = st;
+                break;
+            }
+        }
+    }
+    if (admit_err != LQ_OK) {
+        if (nfinished) *nfinished = nout;
+        return admit_err;
     }
     // 2. one chunk of steps for every resident run
     if (!s->running.empty()) {
@@ -228,7 +263,6 @@ extern "C" lq_status lq_server_tick(lq_server* s, int32_t* finished_ids, int32_t
         }
     }
     // 3. eviction of finished / failed runs
-    int32_t nout = 0;
     std::vector<int32_t> still;
     for (int32_t id : s->running) {
         Request& r = s->reqs[id];

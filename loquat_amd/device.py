@@ -241,6 +241,24 @@ class Engine:
         abi.check(abi.lib().lq_engine_compact(self.h, C.byref(n)))
         return n.value
 
+    def poll(self) -> int:
+        """Collects the device's failure flags; returns the failed resident runs."""
+        n = C.c_int32()
+        abi.check(abi.lib().lq_engine_poll(self.h, C.byref(n)))
+        return n.value
+
+    def failed(self, rid: int) -> bool:
+        f = C.c_int32()
+        abi.check(abi.lib().lq_engine_run_failed(self.h, rid, C.byref(f)))
+        return bool(f.value)
+
+    def set_variant(self, variant: int) -> None:
+        """Interaction variant of this engine (0..6)."""
+        abi.check(abi.lib().lq_engine_set_interaction_variant(self.h, int(variant)))
+
+    def variant(self) -> int:
+        return abi.lib().lq_engine_get_interaction_variant(self.h)
+
     def set_cell_order(self, zbin_bits: int) -> None:
         """0: the reference's in-cell order; b > 0: cells sorted by 2^b z-bins."""
         abi.check(abi.lib().lq_engine_set_cell_order(self.h, int(zbin_bits)))

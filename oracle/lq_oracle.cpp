@@ -422,8 +422,18 @@ void lqo_adami(const lq_params* par, lqo_state* st, const int32_t* beg, const in
 //   a      = sum_j m (s_i/rho_i^2 + s_j/rho_j^2 - Pi_ij I) . grad W + g
 //   Pi_ij  = -alpha cs mu_ij / rho_bar,  mu_ij = h v_ij.x_ij / (r^2 + eta)
 //            when v_ij.x_ij < 0, else 0.
-void lqo_interaction(const lq_params* par, const lqo_state* st, lqo_rates* rt, const int32_t* beg, const int32_t* end,
-                     int32_t* nbr) {
+}  // extern "C"
+
+namespace {
+
+// The interaction loop.  ABS = false: the rates.  ABS = true: for ax, ay,
+// az and drho the sum over the particle's pairs of each pair term's absolute
+// value, plus |g| for the accelerations -- the scale against which the
+// rounding of a sum of cancelling terms is measured (other outputs zero).
+template <bool ABS>
+void interaction_impl(const lq_params* par, const lqo_state* st, lqo_rates* rt, const int32_t* beg,
+                      const int32_t* end, int32_t* nbr) {
+    auto acc = [](double& sum, double term) { sum += ABS ? std::fabs(term) : term; };
     const Grid g = grid_of(par);
     const float i_hf = par->i_h;
     const double i_h = par->i_h, cg = par->kernel_gradient_normalization_par, m = par->m, h = par->h, eta = par->eta;
@@ -467,7 +477,7 @@ void lqo_interaction(const lq_params* par, const lqo_state* st, lqo_rates* rt, c
                     const double gr[3] = {gx, gy, gz};
                     for (int a = 0; a < 3; ++a)
                         for (int bb = 0; bb < 3; ++bb) L[3 * a + bb] += vj * dv[a] * gr[bb];
-                    drho += m * (-(dv[0] * gx + dv[1] * gy + dv[2] * gz));
+                    acc(drho, m * (-(dv[0] * gx + dv[1] * gy + dv[2] * gz)));
                     const double vdx = -(dv[0] * dx + dv[1] * dy + dv[2] * dz);
                     double pij = 0.0;
                     if (vdx < 0.0) {
@@ -481,11 +491,21 @@ void lqo_interaction(const lq_params* par, const lqo_state* st, lqo_rates* rt, c
                     }
                     const T6 S = add(Ti, mul(load6(st->s, j), 1.0 / (rj * rj)));
                     const double sxx = S.xx - pij, syy = S.yy - pij, szz = S.zz - pij;
-                    ax += m * (sxx * gx + S.xy * gy + S.xz * gz);
-                    ay += m * (S.xy * gx + syy * gy + S.yz * gz);
-                    az += m * (S.xz * gx + S.yz * gy + szz * gz);
+                    acc(ax, m * (sxx * gx + S.xy * gy + S.xz * gz));
+                    acc(ay, m * (S.xy * gx + syy * gy + S.yz * gz));
+                    acc(az, m * (S.xz * gx + S.yz * gy + szz * gz));
                 }
             });
+        }
+        if (ABS) {
+            rt->ax[i] = ax + std::fabs((double)par->acc_x);
+            rt->ay[i] = ay + std::fabs((double)par->acc_y);
+            rt->az[i] = az + std::fabs((double)par->acc_z);
+            rt->drho[i] = drho;
+            rt->wx[i] = rt->wy[i] = rt->wz[i] = 0.0;
+            for (int c = 0; c < 6; ++c) rt->d[c][i] = 0.0;
+            if (nbr) nbr[i] = count;
+            continue;
         }
         rt->ax[i] = ax + par->acc_x;
         rt->ay[i] = ay + par->acc_y;
@@ -502,6 +522,23 @@ void lqo_interaction(const lq_params* par, const lqo_state* st, lqo_rates* rt, c
         rt->d[5][i] = 0.5 * (L[5] + L[7]);
         if (nbr) nbr[i] = count;
     }
+}
+
+}  // namespace
+
+extern "C" {
+
+void lqo_interaction(const lq_params* par, const lqo_state* st, lqo_rates* rt, const int32_t* beg, const int32_t* end,
+                     int32_t* nbr) {
+    interaction_impl<false>(par, st, rt, beg, end, nbr);
+}
+
+// Per material particle, sum_j |term_ij| of ax, ay, az (plus |g|) and drho:
+// the scale of the per-particle error of a sum whose terms cancel (an
+// oracle diagnostic).
+void lqo_interaction_abs(const lq_params* par, const lqo_state* st, lqo_rates* rt, const int32_t* beg,
+                         const int32_t* end) {
+    interaction_impl<true>(par, st, rt, beg, end, nullptr);
 }
 
 // Predictor (src/predictor_corrector.cu:169-253): *_ini copy of the material

@@ -49,6 +49,7 @@ def lib() -> C.CDLL:
         L.lqo_permute.argtypes = [C.c_int, ip, C.POINTER(_State), C.POINTER(_Ini)]
         L.lqo_adami.argtypes = [vp, C.POINTER(_State), ip, ip]
         L.lqo_interaction.argtypes = [vp, C.POINTER(_State), C.POINTER(_Rates), ip, ip, ip]
+        L.lqo_interaction_abs.argtypes = [vp, C.POINTER(_State), C.POINTER(_Rates), ip, ip]
         L.lqo_borderline_pairs.argtypes = [vp, C.POINTER(_State), ip, ip, C.POINTER(C.c_int64)]
         L.lqo_predictor.argtypes = [vp, vp, C.POINTER(_State), C.POINTER(_Ini), C.POINTER(_Rates)]
         L.lqo_corrector.argtypes = [vp, vp, C.POINTER(_State), C.POINTER(_Ini), C.POINTER(_Rates)]
@@ -235,6 +236,15 @@ def interaction(par, st: OState, beg, end) -> tuple[ORates, np.ndarray]:
     s, r = st.struct(), rt.struct()
     lib().lqo_interaction(_p(par), C.byref(s), C.byref(r), _i(beg), _i(end), _i(nbr))
     return rt, nbr
+
+
+def interaction_abs(par, st: OState, beg, end) -> ORates:
+    """Per material particle: sum over pairs of |term| for ax, ay, az (plus
+    |g|) and drho -- the scale of a cancelling sum's rounding error."""
+    rt = ORates(st.n)
+    s, r = st.struct(), rt.struct()
+    lib().lqo_interaction_abs(_p(par), C.byref(s), C.byref(r), _i(beg), _i(end))
+    return rt
 
 
 def borderline_pairs(par, st: OState, beg, end) -> tuple[int, int]:

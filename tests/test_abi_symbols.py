@@ -44,3 +44,21 @@ def test_version_and_error_without_gpu():
     assert len(lib.lq_last_error()) > 0
     assert lib.lq_set_interaction_variant(abi.INTERACTION_VARIANTS) == 1
     assert lib.lq_get_interaction_variant() in range(abi.INTERACTION_VARIANTS)
+
+
+def test_particle_capacity_limit_rejected_before_allocation():
+    """Gathers use 32-bit byte offsets (24-byte elements at most): capacities
+    above LQ_MAX_PARTICLES are refused with LQ_ERR_CAPACITY before any device
+    allocation, so this runs without a GPU."""
+    import ctypes as C
+    from loquat_amd import abi
+    lib = abi.lib()
+    assert abi.MAX_PARTICLES * 24 < 2 ** 32 <= (abi.MAX_PARTICLES + 1) * 24
+    h = C.c_void_p()
+    assert lib.lq_engine_create(abi.MAX_PARTICLES + 1, 1000, 1, C.byref(h)) == 8
+    assert lib.lq_grid_create(abi.MAX_PARTICLES + 1, 1000, C.byref(h)) == 8
+    par = abi.LqParams()
+    par.np = abi.MAX_PARTICLES + 1
+    par.npm = par.np
+    # module calls check the particle count before anything else
+    assert lib.lq_particle_interaction(C.byref(par), None, None, None, None, None, None) == 8

@@ -427,3 +427,30 @@ def test_adami_hydrostatic_extrapolation_ignores_wall_stress():
         res.append(O.unpermute(st).s[:, : p.npb].copy())
     changed = res[0][0] != 3e7
     assert np.array_equal(res[0][:, changed], res[1][:, changed])
+
+
+def test_interaction_term_sum_bounds_the_rates():
+    """lqo_interaction_abs: per particle, sum_j |term_ij| (+ |g| for the
+    accelerations) bounds |rate| (triangle inequality), and for a particle
+    with one neighbour equals |rate - g| + |g|."""
+    case = cases.perturb(cases.small_column_3d(nx=8, ny=4, nz=10, seed=3), seed=4)
+    p = case.derived()
+    st = O.OState.from_case(case)
+    O.initial_sort(p, st)
+    g = O.grid_build(p, st)
+    rt, _ = O.interaction(p, st, g["cell_beg"], g["cell_end"])
+    ab = O.interaction_abs(p, st, g["cell_beg"], g["cell_end"])
+    m = slice(p.npb, None)
+    for k in ("ax", "ay", "az", "drho"):
+        assert np.all(getattr(ab, k)[m] >= np.abs(getattr(rt, k)[m]) * (1 - 1e-12)), k
+    assert np.all(ab.az[m] > np.abs(rt.az[m] - p.acc_z))
+    two = _few_particles([[0.05, 0.05, 0.05], [0.0536, 0.0512, 0.0488]], np.zeros((0, 3)), [2000.0, 2100.0],
+                         [[-1e3, -2e3, -3e3, 10, 20, 30], [-4e3, -1e3, -2e3, -5, 7, 9]])
+    p2, st2, g2 = two
+    st2.vx[:] = [0.1, -0.2]
+    r2, nbr = O.interaction(p2, st2, g2["cell_beg"], g2["cell_end"])
+    a2 = O.interaction_abs(p2, st2, g2["cell_beg"], g2["cell_end"])
+    assert list(nbr) == [1, 1]
+    gz = abs(float(p2.acc_z))
+    assert np.allclose(a2.az, np.abs(r2.az - p2.acc_z) + gz, rtol=1e-12)
+    assert np.allclose(a2.drho, np.abs(r2.drho), rtol=1e-12)

@@ -131,3 +131,45 @@ def test_output_errors(tmp_path):
     with pytest.raises(abi.LqError) as e:
         abi.load_restart(str(tmp_path / str(par.output_number)), other)
     assert e.value.status == 5
+
+
+# Column order of a .dat row, as the reference reads it (src/loquat_io.cu:226)
+# and writes it (src/loquat_io.cu:357): pospres xyzw, velrhop xyzw, idep xy,
+# str xx yy zz xy xz yz, stn xx yy zz xy xz yz.
+DAT_COLUMNS = ([("pospres", k) for k in range(4)] + [("velrhop", k) for k in range(4)] +
+               [("idep", k) for k in range(2)] + [("str", k) for k in range(6)] + [("stn", k) for k in range(6)])
+
+
+def _one_particle_par():
+    par = cases.small_column_3d(seed=5).par.copy()
+    par.np, par.npb, par.npm = 1, 0, 1
+    par.output_number = 3
+    return par
+
+
+def test_dat_row_read_in_the_reference_column_order(tmp_path):
+    """A hand-written row of 22 distinct numbers lands field by field where
+    the reference's stream extraction puts it."""
+    values = [float(k + 1) + 0.25 for k in range(22)]
+    (tmp_path / "p.dat").write_text(" ".join(repr(v) for v in values) + "\n")
+    arrs = abi.load_particles(str(tmp_path / "p"), _one_particle_par())
+    for v, (field, k) in zip(values, DAT_COLUMNS):
+        assert arrs[field][0, k] == np.float32(v), (field, k)
+    assert arrs["idep"][0, 2] == 0.0 and arrs["idep"][0, 3] == 0.0   # not in the file
+
+
+def test_dat_row_written_in_the_reference_column_order(tmp_path):
+    """lq_save_output writes one row per particle, 22 space-separated numbers
+    in the reference's order; idep.z and .w are not written."""
+    par = _one_particle_par()
+    arrs = {"pospres": np.array([[1.5, 2.5, 3.5, 4.5]], np.float32),
+            "velrhop": np.array([[5.5, 6.5, 7.5, 8.5]], np.float32),
+            "idep": np.array([[9.5, 10.5, 99.0, 98.0]], np.float32),
+            "str": np.array([[11.5, 12.5, 13.5, 14.5, 15.5, 16.5]], np.float32),
+            "stn": np.array([[17.5, 18.5, 19.5, 20.5, 21.5, 22.5]], np.float32)}
+    abi.save_output(str(tmp_path), par, None, arrs)
+    rows = (tmp_path / "3.dat").read_text().splitlines()
+    assert len(rows) == 1
+    tok = rows[0].split(" ")
+    assert len(tok) == 22
+    assert [float(t) for t in tok] == [k + 1.5 for k in range(22)]

@@ -80,3 +80,27 @@ def test_run_slots_limit_concurrency():
 def test_invalid_config():
     with pytest.raises(abi.LqError):
         Server(0, 10, 1, 10, 1, fake=True)
+
+
+@pytest.mark.parametrize("defect", ["model", "vw", "wall"])
+def test_invalid_request_is_rejected_and_does_not_block_the_queue(defect):
+    """A request that lq_engine_add_run would refuse (constitutive model out
+    of range, model 3 without its constants, a wall particle outside the
+    grid) is rejected at submission; the requests queued behind it run."""
+    a, bad, c = small(6, 1), small(6, 2), small(5, 3)
+    if defect == "model":
+        bad.par.constitutive_model = 7
+    elif defect == "vw":
+        bad.par.constitutive_model = 3
+        bad.vw = abi.LqVw()
+    else:
+        bad.posd[0, 0] = -1.0
+        bad.pospres[0, 0] = -1.0
+    s = Server(10**6, 10**7, 4, 10**12, steps_per_tick=10, fake=True)
+    ia, ib, ic = s.submit(a, 20), s.submit(bad, 20), s.submit(c, 20)
+    assert s.info(ib)["state"] == "rejected"
+    done = []
+    for _ in range(3):
+        done += s.tick()
+    assert sorted(done) == sorted([ia, ic])
+    assert s.info(ia)["state"] == "done" and s.info(ic)["state"] == "done"
