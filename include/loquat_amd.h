@@ -201,6 +201,20 @@ lq_status lq_grid_destroy(lq_grid* grid);
  * (cell_beg = -1 and cell_end = -1 for empty cells). */
 lq_status lq_grid_build(lq_grid* grid, const lq_params* par, const float* pospres, lq_stream stream);
 
+/* No counterpart in the reference.  lq_grid_build for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same cell hash, stable radix sort and
+ * cell table, with results in caller-owned device arrays -- part_cel and
+ * part_idx (int32[np], as lq_grid_arrays gives them) and cell_beg, cell_end
+ * (int32[ncell]) -- and the sort's scratch in a fixed device array inside the
+ * library (about 17 MB per GPU, held until the process ends).  It allocates
+ * nothing.  np above LQ_NOALLOC_MAX_PARTICLES returns LQ_ERR_CAPACITY.  All
+ * calls on one GPU share that scratch: calls that overlap on one GPU (two
+ * streams, two threads) race and must be serialised by the caller. */
+#define LQ_NOALLOC_MAX_PARTICLES 2097152
+lq_status lq_grid_build_noalloc(const lq_params* par, const float* pospres, int32_t* part_cel, int32_t* part_idx,
+                                int32_t* cell_beg, int32_t* cell_end, lq_stream stream);
+
 /* SortArraysPrimary / SortArraysAll (src/loquat.cu:190-213) through
  * SortArrayFloat4 / SortArrayTensor2d (src/loquat_grid.h:62-64):
  * dst[k] = src[part_idx[k]] for every state array; ini_src/ini_dst may be

@@ -90,6 +90,7 @@ INTERACTION_VARIANTS = 7  # LQ_INTERACTION_VARIANTS
 CELL_ORDER_CANDIDATES = 6  # LQ_CELL_ORDER_CANDIDATES
 DEFAULT_INTERACTION_VARIANT = 6  # LQ_DEFAULT_INTERACTION_VARIANT
 MAX_PARTICLES = 178956970  # LQ_MAX_PARTICLES
+NOALLOC_MAX_PARTICLES = 2097152  # LQ_NOALLOC_MAX_PARTICLES
 
 
 class LqError(RuntimeError):
@@ -160,6 +161,7 @@ _PROTOS = {
     "lq_server_stats": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "lq_set_interaction_variant": (i32, [i32]),
     "lq_get_interaction_variant": (i32, []),
+    "lq_grid_build_noalloc": (i32, [C.POINTER(LqParams), vp, vp, vp, vp, vp, vp]),
     "lq_engine_set_interaction_variant": (i32, [vp, i32]),
     "lq_engine_get_interaction_variant": (i32, [vp]),
     "lq_version": (C.c_char_p, []),

@@ -213,6 +213,23 @@ extern "C" lq_status lq_grid_build(lq_grid* grid, const lq_params* par, const fl
     return last_launch("lq_grid_build");
 }
 
+extern "C" lq_status lq_grid_build_noalloc(const lq_params* par, const float* pospres, int32_t* part_cel,
+                                           int32_t* part_idx, int32_t* cell_beg, int32_t* cell_end,
+                                           lq_stream stream) {
+    if (!pospres || !part_cel || !part_idx || !cell_beg || !cell_end) {
+        set_error("lq_grid_build_noalloc: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    LQ_TRY(check_params(par));
+    if (par->np > LQ_NOALLOC_MAX_PARTICLES) {
+        set_error("lq_grid_build_noalloc: np exceeds LQ_NOALLOC_MAX_PARTICLES");
+        return LQ_ERR_CAPACITY;
+    }
+    LQ_TRY(launch_grid_build_noalloc(one_desc(par, nullptr), (const float4*)pospres, part_cel, part_idx, cell_beg,
+                                     cell_end, (hipStream_t)stream));
+    return last_launch("lq_grid_build_noalloc");
+}
+
 extern "C" lq_status lq_grid_permute(lq_grid* grid, int32_t np, const lq_state* src, const lq_state* dst,
                                      const lq_state_ini* ini_src, const lq_state_ini* ini_dst, lq_stream stream) {
     if (!grid || !state_ok(src) || !state_ok(dst) || (ini_src && !ini_ok(ini_src)) ||

@@ -19,6 +19,8 @@
 //   scan    : one workgroup per digit scans that digit's tile counts.
 //   scatter : re-ranks the tile the same way, stages keys/values in LDS in
 //             digit order and writes each digit's run contiguously.
+#include <mutex>
+
 #include "lq_internal.hpp"
 
 namespace lq {
@@ -360,6 +362,72 @@ void launch_grid_build_one(const RunDesc& d, const float4* pospres, GridBuffers&
 void launch_grid_build_table(const RunDesc* d, const int32_t* run_of, const float4* pospres, int n,
                              int64_t ncell_total, int zbits, GridBuffers& g, hipStream_t s) {
     grid_build(DescTable{d, run_of}, pospres, n, rs_bits(ncell_total) + zbits, ncell_total, zbits, g, s);
+}
+
+// ------------------------------------------------- grid build without allocation
+// Scratch of lq_grid_build_noalloc, a file-scope device array: the radix
+// sort's second key/value buffer (the caller's part_cel / part_idx are the
+// first) and its tile histograms and digit totals, for up to
+// NOALLOC_MAX_PARTICLES particles.  Every GPU the library runs on holds its
+// own copy until the process ends.  Nothing in it needs clearing: each pass
+// writes every histogram entry and digit total before reading them.
+constexpr int NOALLOC_TILES = (NOALLOC_MAX_PARTICLES + RS_TILE - 1) / RS_TILE;
+constexpr int64_t NOALLOC_WORDS = 2 * (int64_t)NOALLOC_MAX_PARTICLES + (int64_t)RS_BINS * NOALLOC_TILES + RS_BINS;
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+
+namespace {
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[64] = {};  // per device: the device address of scratch
+}  // namespace
+
+// Device address of this GPU's scratch, looked up once per device with that
+// device current.  The host-side name of the array is not a device address.
+static lq_status noalloc_scratch(uint32_t** out) {
+    int dev = 0;
+    LQ_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) {
+        set_error("lq_grid_build_noalloc: device index beyond 63");
+        return LQ_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        LQ_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (uint32_t*)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return LQ_OK;
+}
+
+lq_status launch_grid_build_noalloc(const RunDesc& d, const float4* pospres, int32_t* part_cel, int32_t* part_idx,
+                                    int32_t* cell_beg, int32_t* cell_end, hipStream_t s) {
+    if (d.np > NOALLOC_MAX_PARTICLES) {
+        set_error("lq_grid_build_noalloc: np exceeds LQ_NOALLOC_MAX_PARTICLES");
+        return LQ_ERR_CAPACITY;
+    }
+    uint32_t* ptr = nullptr;
+    LQ_TRY(noalloc_scratch(&ptr));
+    // The sorted pairs end in slot passes % 2: make that slot the caller's
+    // arrays and the other one the scratch.
+    const int bits = rs_bits(d.par.ncell) + d.zbits;
+    const int out = ((bits + 7) / 8) % 2;
+    GridBuffers g{};
+    g.keys[out] = (uint32_t*)part_cel;
+    g.vals[out] = (uint32_t*)part_idx;
+    g.keys[1 - out] = ptr;
+    g.vals[1 - out] = ptr + NOALLOC_MAX_PARTICLES;
+    g.hist = ptr + 2 * (int64_t)NOALLOC_MAX_PARTICLES;
+    g.totals = g.hist + (int64_t)RS_BINS * NOALLOC_TILES;
+    g.cell_beg = cell_beg;
+    g.cell_end = cell_end;
+    g.np_cap = NOALLOC_MAX_PARTICLES;
+    g.ncell_cap = d.par.ncell;
+    launch_grid_build_one(d, pospres, g, s);
+    if (d.np > 0 && g.sorted_slot != out) {
+        set_error("lq_grid_build_noalloc: sorted result not in the caller's arrays");
+        return LQ_ERR_STATE;
+    }
+    return LQ_OK;
 }
 
 void launch_permute(const uint32_t* perm, int n, const DevState& src, const DevState& dst, const DevIni* ini_src,

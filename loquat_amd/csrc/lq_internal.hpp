@@ -67,6 +67,13 @@ void free_grid(GridBuffers& g);
 void launch_grid_build_one(const RunDesc& d, const float4* pospres, GridBuffers& g, hipStream_t s);
 void launch_grid_build_table(const RunDesc* d, const int32_t* run_of, const float4* pospres, int n,
                              int64_t ncell_total, int zbits, GridBuffers& g, hipStream_t s);
+// The grid build of one run into caller-owned arrays (part_cel, part_idx:
+// np entries; cell_beg, cell_end: ncell), the sort's scratch in a
+// file-scope device array of lq_grid.hip (np <= NOALLOC_MAX_PARTICLES).  No
+// allocation; calls that overlap on one GPU share that scratch and race.
+constexpr int NOALLOC_MAX_PARTICLES = LQ_NOALLOC_MAX_PARTICLES;
+lq_status launch_grid_build_noalloc(const RunDesc& d, const float4* pospres, int32_t* part_cel, int32_t* part_idx,
+                                    int32_t* cell_beg, int32_t* cell_end, hipStream_t s);
 // dst[k] = src[perm[k]] for all state arrays (and ini when both set).  When
 // str_dst is non-null the permuted stress goes there instead of dst.str.
 void launch_permute(const uint32_t* perm, int n, const DevState& src, const DevState& dst, const DevIni* ini_src,
