@@ -18,14 +18,19 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The CPU baseline's OpenMP threads stay on neighbouring cores (read by the
+# OpenMP runtime when the oracle library loads).
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 METRIC = "M particle-steps/s at 1M particles (1 GPU); ensemble p50/p99 vs plain loop"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3       # published FP32 vector peak (packed FP32 instructions)
 FP32_VALU_PEAK_TFLOPS = 78.6   # non-packed FP32 vector (packed FP32 is disabled in the build)
 FLOPS_PER_PAIR = 80            # algorithmic FP32 flops per neighbour pair
 BYTES_READ_PER_PARTICLE = 64   # x (FP64, 24) + v (12) + rho (4) + stress (24)
@@ -33,24 +38,47 @@ BYTES_WRITE_PER_MATERIAL = 52  # a (12) + drho (4) + spin (12) + strain rate (24
 BYTES_PER_CELL = 8             # cell_beg + cell_end
 WORKLOAD_ID = "column_3d dx=5mm npm=1024000"  # the key of this workload in profiles/*.json
 KERNEL_NAMES = {0: "k_interaction_direct", 1: "k_interaction_lds", 2: "k_interaction_unrolled",
-                3: "k_interaction_v3<DescTable, 1, false, false, false>",
-                4: "k_interaction_v3<DescTable, 2, false, false, false>",
-                5: "k_interaction_v3<DescTable, 1, true, false, false>",
-                6: "k_interaction_v3<DescTable, 2, true, false, false>"}
-# The corrector half scans the predictor half's cell table with this kernel
-# (variant 6's shape), whatever variant the predictor half uses.
-STALE_KERNEL = "k_interaction_v3<DescTable, 2, true, false, true>"
+                3: "k_interaction_v3<DescTable, 1, false, false, false, 40, 4, 4>",
+                4: "k_interaction_v3<DescTable, 2, false, false, false, 40, 4, 4>",
+                5: "k_interaction_v3<DescTable, 1, true, false, false, 40, 4, 4>",
+                6: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 4>",
+                7: "k_interaction_v3<DescTable, 2, true, false, false, 32, 4, 4>",
+                8: "k_interaction_v3<DescTable, 2, true, false, false, 48, 4, 4>",
+                9: "k_interaction_v3<DescTable, 2, true, false, false, 40, 8, 4>",
+                10: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 2>",
+                11: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1>"}
+# The corrector half scans the predictor half's cell table with variant 6's
+# shape, in the workgroup size of variants 10 (128 threads) and 11 (64).
+def stale_kernel(variant: int) -> str:
+    return f"k_interaction_v3<DescTable, 2, true, false, true, 40, 4, {({10: 2, 11: 1}).get(variant, 4)}>"
 
 
 def roofline_kernel(variant: int) -> str:
-    return f"{KERNEL_NAMES[variant]} (predictor half) + {STALE_KERNEL} (corrector half)"
+    return f"{KERNEL_NAMES[variant]} (predictor half) + {stale_kernel(variant)} (corrector half)"
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(case, steps: int) -> dict:
+def host_info() -> dict:
+    """CPU of this host, as nproc and lscpu report it (read before the GPU is
+    touched)."""
+    info = {"nproc": len(os.sched_getaffinity(0)), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)",
+                             "CPU max MHz"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(case, steps: int, host: dict) -> dict:
     """FP64 oracle (C++/OpenMP) on this host: initial sort, 2 warm-up steps,
     then `steps` timed steps of the same workload on all threads, and one
     more step on one thread."""
@@ -69,25 +97,46 @@ def cpu_baseline(case, steps: int) -> dict:
     O.run(p, case.vw, st, 1, 2 + steps)
     dt1 = time.perf_counter() - t1
     O.set_num_threads(threads)
-    cpu_model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
     return {
         "value": case.npm * steps / dt / 1e6,
         "unit": "M particle-steps/s",
         "cores": threads,
         "kind": "port",
         "sample": f"FP64 oracle, same 1.55 M-particle column: 2 warm-up + {steps} timed full steps "
-                  f"({dt:.1f} s), OpenMP threads={threads}, {cpu_model}",
+                  f"({dt:.1f} s), OpenMP threads={threads} bound close, {host.get('Model name', '')}",
         "s_per_step": dt / steps,
         "one_thread_s_per_step": dt1,
+        "host": host,
     }
+
+
+def cpu_baseline_c1(steps: int, one_thread_steps: int) -> dict:
+    """The plane-strain sand column (5,000 material particles), all of its
+    steps on the FP64 oracle with every OpenMP thread, and a bounded sample on
+    one thread."""
+    from loquat_amd import cases
+    from oracle import lq_oracle as O
+
+    case = cases.column_2d_small(nsteps=steps)
+    p = case.derived()
+    st = O.OState.from_case(case)
+    O.initial_sort(p, st)
+    t0 = time.perf_counter()
+    O.run(p, case.vw, st, steps, 0)
+    dt = time.perf_counter() - t0
+    threads = O.num_threads()
+    O.set_num_threads(1)
+    st1 = O.OState.from_case(case)
+    O.initial_sort(p, st1)
+    t1 = time.perf_counter()
+    O.run(p, case.vw, st1, one_thread_steps, 0)
+    dt1 = time.perf_counter() - t1
+    O.set_num_threads(threads)
+    return {"workload": f"plane-strain sand column 50x100x1 at dx=2 mm, npm={case.npm}, np={case.np}, FP64",
+            "steps": steps, "seconds": round(dt, 3), "threads": threads,
+            "M_particle_steps_s": round(case.npm * steps / dt / 1e6, 4),
+            "one_thread": {"steps": one_thread_steps, "s_per_step": round(dt1 / one_thread_steps, 6),
+                           "M_particle_steps_s": round(case.npm * one_thread_steps / dt1 / 1e6, 4)}}
 
 
 def measured_traffic(kernel: str) -> tuple[float | None, str | None]:
@@ -106,6 +155,22 @@ def measured_traffic(kernel: str) -> tuple[float | None, str | None]:
         if j.get("kernel") == kernel and j.get("workload") == WORKLOAD_ID:
             return float(j["traffic_bytes"]), "profiles/" + f
     return None, None
+
+
+def measured_launch_us(kernel: str) -> float | None:
+    """Mean launch time of `kernel` on this workload in the committed
+    rocprofv3 kernel trace (profiles/*.json), beside the live event timing."""
+    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+    if not os.path.isdir(d):
+        return None
+    for f in sorted(os.listdir(d)):
+        if not f.endswith(".json"):
+            continue
+        with open(os.path.join(d, f)) as fh:
+            j = json.load(fh)
+        if j.get("kernel") == kernel and j.get("workload") == WORKLOAD_ID and "avg_us" in j:
+            return float(j["avg_us"])
+    return None
 
 
 def pair_count(case) -> int:
@@ -194,7 +259,9 @@ def ensemble_section(nruns: int, steps: int, warm: int, seed: int = 0) -> dict:
         "plain_loop_M_particle_steps_s": round(plain, 3),
         "speedup_vs_plain_loop": round(batched / plain, 3),
         "failed_runs": failed,
-        "batched_completion_s": {"p50": round(t_batch, 4), "p99": round(t_batch, 4)},
+        "batched_completion_s": {"p50": round(t_batch, 4), "p99": round(t_batch, 4),
+                                 "note": "every run has the same step count, so all runs of the batch "
+                                         "complete together: p50 = p99 is the measured completion time"},
         "plain_loop_completion_s": {"p50": round(_pct(completion, 50), 4), "p99": round(_pct(completion, 99), 4)},
     }
 
@@ -221,6 +288,7 @@ def served_section(nreq: int, step_scale: float, seed: int = 0) -> dict:
     done_at, ids, fin = {}, {}, {}
     t0 = time.perf_counter()
     nxt = 0
+    busy = 0.0   # time inside ticks with runs resident: the server's device-busy time
     while len(done_at) < nreq:
         now = time.perf_counter() - t0
         while nxt < nreq and reqs[nxt][0] <= now:
@@ -230,7 +298,10 @@ def served_section(nreq: int, step_scale: float, seed: int = 0) -> dict:
             if nxt < nreq:
                 time.sleep(max(0.0, min(0.01, reqs[nxt][0] - now)))
             continue
-        for rid in srv.tick():
+        tt = time.perf_counter()
+        finished = srv.tick()
+        busy += time.perf_counter() - tt
+        for rid in finished:
             done_at[ids[rid]] = time.perf_counter() - t0
             fin[ids[rid]] = srv.info(rid)
     makespan = time.perf_counter() - t0
@@ -241,32 +312,44 @@ def served_section(nreq: int, step_scale: float, seed: int = 0) -> dict:
     # with the steps it took
     work = sum(reqs[i][1].npm * fin[i]["steps"] for i in range(nreq))
     failed = sum(fin[i]["state"] != "done" for i in range(nreq))
-    # plain loop: FIFO, one run at a time on the same engine code
+    # plain loop: FIFO, one run at a time on the same engine code; each run's
+    # time covers the same work a tick does for it (upload and first sort,
+    # its steps, removal), so both throughputs are work / device-busy time
     big = max(reqs, key=lambda r: r[1].np)[1]
     eng = Engine(big.np, max(c.derived().ncell for _, c, _ in reqs), 1)
-    clock, lat_plain, busy, work_plain = 0.0, [], 0.0, 0
+    clock, lat_plain, busy_plain, work_plain = 0.0, [], 0.0, 0
     for a, c, s in reqs:
-        rid = eng.add(c, s)
-        eng.sync()
         t1 = time.perf_counter()
+        rid = eng.add(c, s)
         eng.step(s)
         eng.sync()
-        dt = time.perf_counter() - t1
-        work_plain += c.npm * eng.info(rid)["steps"]
+        steps_done = eng.info(rid)["steps"]
         eng.remove(rid)
-        busy += dt
+        dt = time.perf_counter() - t1
+        work_plain += c.npm * steps_done
+        busy_plain += dt
         clock = max(clock, a) + dt
         lat_plain.append(clock - a)
     eng.close()
+    served = work / busy / 1e6
+    plain = work_plain / busy_plain / 1e6
+    big_work = sum(c.npm * fin[i]["steps"] for i, (_, c, _) in enumerate(reqs) if c.npm >= 2e5)
     return {
         "workload": f"{nreq} requests, Poisson 20/s, column DP / slope von Wolffersdorff 50/50, "
                     f"npm log-normal clipped [5k,500k], steps U[2000,8000] x {step_scale}",
         "latency_s": {"p50": round(_pct(lat, 50), 4), "p99": round(_pct(lat, 99), 4)},
         "plain_loop_latency_s": {"p50": round(_pct(lat_plain, 50), 4), "p99": round(_pct(lat_plain, 99), 4)},
-        "aggregate_M_particle_steps_s": round(work / makespan / 1e6, 3),
-        "plain_loop_M_particle_steps_s": round(work_plain / busy / 1e6, 3),
+        "busy_M_particle_steps_s": round(served, 3),
+        "plain_loop_M_particle_steps_s": round(plain, 3),
+        "speedup_vs_plain_loop": round(served / plain, 3),
+        "throughput_definition": "material particle-steps / device-busy time for both (server: time inside "
+                                 "ticks with runs resident; plain loop: add + steps + remove of each run)",
+        "makespan_M_particle_steps_s": round(work / makespan / 1e6, 3),
+        "work_share_runs_over_200k": round(big_work / max(work, 1), 4),
+        "np_per_npm": round(sum(c.np for _, c, _ in reqs) / sum(c.npm for _, c, _ in reqs), 3),
         "failed_runs": failed,
         "makespan_s": round(makespan, 3),
+        "busy_s": round(busy, 3),
     }
 
 
@@ -329,15 +412,24 @@ def main() -> None:
     ap.add_argument("--no-extra", action="store_true", help="skip the ensemble / served / slope sections")
     ap.add_argument("--slope-only", action="store_true",
                     help="time only the embankment section and print its JSON (for profiling that workload)")
+    ap.add_argument("--served-only", action="store_true",
+                    help="run only the served section (with --serve-requests / --serve-step-scale) and print its JSON")
     ap.add_argument("--cell-order", type=int, default=4, metavar="B",
                     help="in-cell order of the step's sort: 0 = by cell only, B > 0 = 2^B z-bins per cell")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="after the run, 5 device copies of MB MiB (reference kernel for HBM byte counters)")
+    ap.add_argument("--c1-steps", type=int, default=20000,
+                    help="steps of the plane-strain column timed on the FP64 oracle (all of them: 20000)")
+    ap.add_argument("--c1-one-thread-steps", type=int, default=1000)
+    ap.add_argument("--no-ref-order", action="store_true",
+                    help="skip timing the column again in the reference's in-cell order")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the SPH step of one run lives on one device "
               "(ensembles shard by whole runs with no communication); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    host = host_info()   # before anything touches the GPU
 
     import torch
 
@@ -350,11 +442,15 @@ def main() -> None:
         sec = slope_section(a.slope_steps, autotune=bool(a.autotune), cell_order=a.cell_order)
         print(json.dumps({"slope_vw": sec}), flush=True)
         return
+    if a.served_only:
+        print(json.dumps({"served": served_section(a.serve_requests, a.serve_step_scale)}), flush=True)
+        return
     if a.variant is None:
         a.variant = abi.DEFAULT_INTERACTION_VARIANT
     abi.check(abi.lib().lq_set_interaction_variant(a.variant))
 
-    case = cases.column_3d(nsteps=a.warmup + a.steps + a.profile_steps + 1)
+    # room for the timed steps twice: in the tuned order and in the reference order
+    case = cases.column_3d(nsteps=2 * (a.warmup + a.steps + a.profile_steps) + 1)
     p = case.derived()
     log(f"workload: np={case.np} npm={case.npm} npb={p.npb} ncell={p.ncell} dt={p.dt:.4e}")
 
@@ -402,6 +498,27 @@ def main() -> None:
     # same stream and state, right after the timed loop).
     stage = eng.profile(a.profile_steps)
     info = eng.info(rid)
+    # The same run timed again in the reference's in-cell order (sorted by
+    # cell only, SortParticleByCellIndex): the cost of that exact order.
+    ref_order = None
+    if not a.no_ref_order:
+        eng.set_cell_order(0)
+        eng.step(a.warmup)
+        eng.sync()
+        ev2, ev3 = hip.Event(enable_timing=True), hip.Event(enable_timing=True)
+        ev2.record(s)
+        eng.step(a.steps)
+        ev3.record(s)
+        eng.sync()
+        ms_ref = ev2.elapsed_time(ev3) / a.steps
+        stage_ref = eng.profile(a.profile_steps)
+        ref_order = {"value": round(case.npm / (ms_ref * 1e-3) / 1e6, 3), "ms_per_step": round(ms_ref, 4),
+                     "cell_order_zbits": 0,
+                     "stage_ms_per_step": {k: round(v, 4) for k, v in
+                                           zip(("grid", "adami", "interaction", "integrate", "shepard"), stage_ref)}}
+        log(f"reference in-cell order: {ref_order}")
+        if eng.info(rid)["steps"] < info["steps"] + a.warmup + a.steps:
+            raise SystemExit("bench: the column failed in the reference in-cell order")
     eng.close()
     inter_launch_ms = stage[2] / 2.0   # two interaction launches per step
     log("stage ms/step: grid %.3f adami %.3f interaction %.3f integrate %.3f shepard %.3f" % tuple(stage))
@@ -411,6 +528,13 @@ def main() -> None:
     gbs = bytes_launch / (inter_launch_ms * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(roofline_kernel(variant))
     tflops = flops_launch / (inter_launch_ms * 1e-3) / 1e12
+    # the binding bound: the larger of the launch's HBM time and its FP32 time
+    # at the published peak (VALU for these algorithmic counts)
+    t_hbm = bytes_launch / (HBM_PEAK_GBS * 1e9)
+    t_valu = flops_launch / (FP32_PEAK_TFLOPS * 1e12)
+    t_valu_np = flops_launch / (FP32_VALU_PEAK_TFLOPS * 1e12)
+    launch_s = inter_launch_ms * 1e-3
+    trace_us = measured_launch_us(roofline_kernel(variant))
 
     value = case.npm * a.steps / (gpu_ms * 1e-3) / 1e6
     out = {
@@ -441,25 +565,27 @@ def main() -> None:
             "kernel": roofline_kernel(variant),
             "measured_by": "HIP events around the interaction launches on the engine stream (lq_engine_profile); "
                            "launch_ms is the mean of the step's two launches",
-            "bound": "hbm",
-            "achieved": round(gbs, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 5),
+            "bound": "valu",
+            "achieved": round(tflops, 3),
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(max(t_hbm, t_valu) / launch_s, 5),
+            "frac_definition": "max(bytes / 8 TB/s, flops / 157.3 TFLOP/s) / launch time (FP32 flops bind)",
+            "frac_nonpacked": round(max(t_hbm, t_valu_np) / launch_s, 5),
+            "peak_nonpacked": FP32_VALU_PEAK_TFLOPS,
             "traffic": traffic,
             "traffic_source": traffic_src,
             "launch_ms": round(inter_launch_ms, 5),
+            "launch_us_rocprof_trace": trace_us,
+            "flops_per_launch": flops_launch,
+            "pairs": pairs,
             "bytes_per_launch": bytes_launch,
-            "valu": {
-                "flops_per_launch": flops_launch,
-                "pairs": pairs,
-                "achieved_tflops": round(tflops, 3),
-                "peak_tflops": FP32_VALU_PEAK_TFLOPS,
-                "frac": round(tflops / FP32_VALU_PEAK_TFLOPS, 5),
-            },
+            "hbm": {"achieved_gbs": round(gbs, 2), "peak_gbs": HBM_PEAK_GBS,
+                    "frac": round(gbs / HBM_PEAK_GBS, 5)},
         },
         "stage_ms_per_step": {k: round(v, 4) for k, v in
                               zip(("grid", "adami", "interaction", "integrate", "shepard"), stage)},
+        "reference_in_cell_order": ref_order,
         "autotune": tuned,
         "cpu_baseline": None,
     }
@@ -486,7 +612,11 @@ def main() -> None:
         log(f"slope: {out['slope_vw']} ({time.perf_counter() - t:.1f} s)")
     if not a.no_cpu:
         log(f"CPU baseline: FP64 oracle, {a.cpu_steps} steps ...")
-        out["cpu_baseline"] = cpu_baseline(cases.column_3d(nsteps=a.cpu_steps + 3), a.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(cases.column_3d(nsteps=a.cpu_steps + 3), a.cpu_steps, host)
+        if a.c1_steps > 0:
+            t = time.perf_counter()
+            out["cpu_baseline_c1"] = cpu_baseline_c1(a.c1_steps, a.c1_one_thread_steps)
+            log(f"CPU baseline C1: {out['cpu_baseline_c1']} ({time.perf_counter() - t:.1f} s)")
     print(json.dumps(out), flush=True)
 
 

@@ -403,8 +403,8 @@ lq_status lq_engine_profile(lq_engine* eng, int32_t nsteps, double* ms_stage5);
  * the same neighbour sets and sum in the same order).  ms_per_variant
  * (nullable, LQ_INTERACTION_VARIANTS entries) receives the mean launch time
  * of each variant. */
-#define LQ_INTERACTION_VARIANTS 7
-#define LQ_DEFAULT_INTERACTION_VARIANT 6
+#define LQ_INTERACTION_VARIANTS 12
+#define LQ_DEFAULT_INTERACTION_VARIANT 11
 lq_status lq_engine_autotune(lq_engine* eng, int32_t nrep, int32_t* best, double* ms_per_variant);
 
 /* Picks the in-cell order (lq_engine_set_cell_order: 2^b height bins per
@@ -432,8 +432,16 @@ lq_status lq_engine_autotune_cell_order(lq_engine* eng, int32_t nrep, int32_t* b
  * 4 = variant 3 evaluating the lists two pairs at a time,
  * 5, 6 = variants 3, 4 where a lane past the end of its list loads nothing
  *        (exec-masked gathers) instead of re-reading its own particle
- *        (6 is the default).
- * Variants 3 and up also select the list-based Adami and Shepard scans. */
+ *        (the default before variant 11),
+ * 7 - 10 = variant 6's code in other shapes, for the autotuner: 7 with
+ *        32-entry neighbour lists, 8 with 48-entry lists (three workgroups
+ *        per CU instead of four), 9 testing eight candidates per load round,
+ *        10 in 128-thread workgroups, 11 in 64-thread workgroups (the
+ *        default: one wave per workgroup, so a CU holds as many as its
+ *        registers allow and a finished wave frees its slot at once).
+ * Variants 3 and up also select the list-based Adami and Shepard scans; the
+ * corrector half's stale-table scan and the artificial-pressure kernel use
+ * variant 6's shape (in the workgroup size of variants 10 and 11). */
 lq_status lq_set_interaction_variant(int32_t variant);
 int32_t lq_get_interaction_variant(void);
 lq_status lq_engine_set_interaction_variant(lq_engine* eng, int32_t variant);

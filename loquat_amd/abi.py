@@ -86,9 +86,9 @@ class LqServerConfig(C.Structure):
 
 
 REQ_STATE = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "rejected"}
-INTERACTION_VARIANTS = 7  # LQ_INTERACTION_VARIANTS
+INTERACTION_VARIANTS = 12  # LQ_INTERACTION_VARIANTS
 CELL_ORDER_CANDIDATES = 6  # LQ_CELL_ORDER_CANDIDATES
-DEFAULT_INTERACTION_VARIANT = 6  # LQ_DEFAULT_INTERACTION_VARIANT
+DEFAULT_INTERACTION_VARIANT = 11  # LQ_DEFAULT_INTERACTION_VARIANT
 MAX_PARTICLES = 178956970  # LQ_MAX_PARTICLES
 NOALLOC_MAX_PARTICLES = 2097152  # LQ_NOALLOC_MAX_PARTICLES
 

@@ -42,9 +42,10 @@ struct RunDesc {
 
 // Descriptor access for one run passed by value (module-level calls).
 // `padded`: the particle arrays carry SCAN_PAD spare entries past the last
-// particle, so a neighbour scan may read up to three candidates beyond the end
-// of a range without clamping (engine buffers; module calls get caller arrays).
-constexpr int SCAN_PAD = 4;
+// particle, so a neighbour scan reading blocks of up to SCAN_PAD candidates
+// may read past the end of a range without clamping (engine buffers; module
+// calls get caller arrays).
+constexpr int SCAN_PAD = 8;
 
 struct DescOne {
     static constexpr bool padded = false;

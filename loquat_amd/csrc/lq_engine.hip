@@ -156,7 +156,7 @@ extern "C" lq_status lq_neighbour_r2_cut(const lq_params* par, float* r2_cut) {
 
 extern "C" lq_status lq_set_interaction_variant(int32_t v) {
     if (v < 0 || v >= LQ_INTERACTION_VARIANTS) {
-        set_error("interaction variant must be 0..6");
+        set_error("interaction variant must be 0 .. LQ_INTERACTION_VARIANTS - 1");
         return LQ_ERR_INVALID_ARG;
     }
     g_variant = v;
@@ -762,7 +762,7 @@ extern "C" lq_status lq_engine_set_graphs(lq_engine* e, int32_t use_graphs) {
 
 extern "C" lq_status lq_engine_set_interaction_variant(lq_engine* e, int32_t variant) {
     if (!e || variant < 0 || variant >= LQ_INTERACTION_VARIANTS) {
-        set_error("lq_engine_set_interaction_variant: variant must be 0..6");
+        set_error("lq_engine_set_interaction_variant: variant must be 0 .. LQ_INTERACTION_VARIANTS - 1");
         return LQ_ERR_INVALID_ARG;
     }
     LQ_HIP_TRY(hipStreamSynchronize(e->stream));

@@ -669,13 +669,14 @@ __device__ __forceinline__ bool in_current_stencil(const ScanGeom& g, float4 ps,
            abs(z - cell_coord(ps.z, g.i_h)) <= 1;
 }
 
-template <int B, bool PADDED, int CAP = IC2_CAP, bool STALE = false, class Eval>
+template <int B, bool PADDED, int CAP = IC2_CAP, bool STALE = false, int UU = IC2_U, class Eval>
 __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer_hi, const ScanGeom& g, float4 ps,
                                             float r2cut, const int32_t* __restrict__ beg,
                                             const int32_t* __restrict__ end, const float4* __restrict__ pp,
                                             int32_t (*my)[64], int lane, uint32_t dummy, Eval&& eval) {
     static_assert(B == 1 || B == 2, "batch of 1 or 2");
-    constexpr int U = IC2_U;  // candidates per load round (SCAN_PAD covers a block past a range's end)
+    static_assert(UU <= SCAN_PAD, "a block of candidates may run SCAN_PAD entries past a range");
+    constexpr int U = UU;  // candidates per load round
     int count = 0, cnt = 0;
     auto flush = [&]() {
         // bottom-tested loop: the accumulators flow straight from one
@@ -910,11 +911,15 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
 // evaluates the lists two pairs at a time; same pairs, same order.  ART adds
 // the artificial pressure; STALE scans a table built before the particles
 // last moved (sn), held to four waves per SIMD (at most 128 VGPRs).
-template <class DP, int B, bool MASKED = false, bool ART = false, bool STALE = false>
+// Shape parameters searched by the autotuner: CAP list entries per lane,
+// UU candidates per load round, WPB waves per workgroup (lists are per wave,
+// so a workgroup holds WPB x CAP x 64 entries of LDS).
+template <class DP, int B, bool MASKED = false, bool ART = false, bool STALE = false, int CAP = IC3_CAP,
+          int UU = IC2_U, int WPB = 4>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(STALE ? 4 : 1))) void
 k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ beg, const int32_t* __restrict__ end,
                  IndexMap mats, int n, int32_t* __restrict__ nbr, StaleIn sn = {}) {
-    __shared__ int32_t lists[4][IC3_CAP][64];
+    __shared__ int32_t lists[WPB][CAP][64];
     const int t = xcd_thread();
     const int k = t < n ? mats(t) : -1;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -965,11 +970,11 @@ k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ be
         }
     };
     if constexpr (B == 1) {
-        count = scan_compact<1, DP::padded, IC3_CAP, STALE>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+        count = scan_compact<1, DP::padded, CAP, STALE, UU>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
                                             mat ? (uint32_t)k : 0u,
                                             [&](uint32_t j, bool act) { pair_compute<ART>(pi, load(j, act), c, a); });
     } else {
-        count = scan_compact<2, DP::padded, IC3_CAP, STALE>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+        count = scan_compact<2, DP::padded, CAP, STALE, UU>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
                                             mat ? (uint32_t)k : 0u,
                                             [&](uint32_t j0, bool a0, uint32_t j1, bool a1) {
                                                 const PairIn q0 = load(j0, a0);
@@ -1650,11 +1655,28 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
                                const StaleIn* stale, hipStream_t s) {
     if (n <= 0) return;
     const int nb = blocks_for(n, 256);
-    if (stale && art)  // stale tables: variant 6's shape
+    // stale tables: variant 6's shape, in the workgroup size of variants 10
+    // and 11
+    if (stale && art && variant == 11)
+        k_interaction_v3<DP, 2, true, true, true, IC3_CAP, IC2_U, 1>
+            <<<blocks_for(n, 64), 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
+    else if (stale && variant == 11)
+        k_interaction_v3<DP, 2, true, false, true, IC3_CAP, IC2_U, 1>
+            <<<blocks_for(n, 64), 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
+    else if (stale && art && variant == 10)
+        k_interaction_v3<DP, 2, true, true, true, IC3_CAP, IC2_U, 2>
+            <<<blocks_for(n, 128), 128, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
+    else if (stale && variant == 10)
+        k_interaction_v3<DP, 2, true, false, true, IC3_CAP, IC2_U, 2>
+            <<<blocks_for(n, 128), 128, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
+    else if (stale && art)
         k_interaction_v3<DP, 2, true, true, true><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
     else if (stale)
         k_interaction_v3<DP, 2, true, false, true><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
-    else if (art)  // artificial pressure: one kernel (variant 6's shape) carries the extra term
+    else if (art && variant == 11)  // artificial pressure: variant 6's shape carries the extra term
+        k_interaction_v3<DP, 2, true, true, false, IC3_CAP, IC2_U, 1>
+            <<<blocks_for(n, 64), 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+    else if (art)
         k_interaction_v3<DP, 2, true, true><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else if (variant == 0)
         k_interaction_direct<DP><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
@@ -1668,6 +1690,18 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
         k_interaction_v3<DP, 2><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else if (variant == 5)
         k_interaction_v3<DP, 1, true><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+    else if (variant == 7)  // variant 6 with 32-entry lists
+        k_interaction_v3<DP, 2, true, false, false, 32><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+    else if (variant == 8)  // 48-entry lists (three workgroups per CU)
+        k_interaction_v3<DP, 2, true, false, false, 48><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+    else if (variant == 9)  // eight candidates per load round
+        k_interaction_v3<DP, 2, true, false, false, IC3_CAP, 8><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+    else if (variant == 10)  // 128-thread workgroups
+        k_interaction_v3<DP, 2, true, false, false, IC3_CAP, IC2_U, 2>
+            <<<blocks_for(n, 128), 128, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+    else if (variant == 11)  // 64-thread workgroups
+        k_interaction_v3<DP, 2, true, false, false, IC3_CAP, IC2_U, 1>
+            <<<blocks_for(n, 64), 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else
         k_interaction_v3<DP, 2, true><<<nb, 256, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
 }

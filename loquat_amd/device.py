@@ -253,7 +253,7 @@ class Engine:
         return bool(f.value)
 
     def set_variant(self, variant: int) -> None:
-        """Interaction variant of this engine (0..6)."""
+        """Interaction variant of this engine (0 .. abi.INTERACTION_VARIANTS - 1)."""
         abi.check(abi.lib().lq_engine_set_interaction_variant(self.h, int(variant)))
 
     def variant(self) -> int:
