@@ -139,11 +139,9 @@ def cpu_baseline_c1(steps: int, one_thread_steps: int) -> dict:
                            "M_particle_steps_s": round(case.npm * one_thread_steps / dt1 / 1e6, 4)}}
 
 
-def measured_traffic(kernel: str) -> tuple[float | None, str | None]:
-    """HBM bytes per launch of `kernel` on this workload, from the committed
-    rocprofv3 summary of the same command (FETCH_SIZE + WRITE_SIZE per
-    dispatch, each scaled by what the counter reads for a device copy of
-    known size), or None when no summary covers this kernel."""
+def profile_entry(kernel: str) -> tuple[dict | None, str | None]:
+    """The committed rocprofv3 summary (profiles/*.json) of `kernel` on this
+    workload, and its path, or (None, None)."""
     d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
     if not os.path.isdir(d):
         return None, None
@@ -153,24 +151,33 @@ def measured_traffic(kernel: str) -> tuple[float | None, str | None]:
         with open(os.path.join(d, f)) as fh:
             j = json.load(fh)
         if j.get("kernel") == kernel and j.get("workload") == WORKLOAD_ID:
-            return float(j["traffic_bytes"]), "profiles/" + f
+            return j, "profiles/" + f
     return None, None
+
+
+def measured_traffic(kernel: str) -> tuple[float | None, str | None]:
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 summary of the same command (FETCH_SIZE + WRITE_SIZE per
+    dispatch, each scaled by what the counter reads for a device copy of
+    known size), or None when no summary covers this kernel."""
+    j, src = profile_entry(kernel)
+    return (float(j["traffic_bytes"]), src) if j else (None, None)
 
 
 def measured_launch_us(kernel: str) -> float | None:
     """Mean launch time of `kernel` on this workload in the committed
     rocprofv3 kernel trace (profiles/*.json), beside the live event timing."""
-    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
-    if not os.path.isdir(d):
+    j, _ = profile_entry(kernel)
+    return float(j["avg_us"]) if j and "avg_us" in j else None
+
+
+def measured_l1(kernel: str) -> dict | None:
+    """Vector L1 (TCP) cache-line accesses per launch and per CU per clock of
+    `kernel` on this workload, from the committed counter summary."""
+    j, src = profile_entry(kernel)
+    if not j or "l1" not in j:
         return None
-    for f in sorted(os.listdir(d)):
-        if not f.endswith(".json"):
-            continue
-        with open(os.path.join(d, f)) as fh:
-            j = json.load(fh)
-        if j.get("kernel") == kernel and j.get("workload") == WORKLOAD_ID and "avg_us" in j:
-            return float(j["avg_us"])
-    return None
+    return dict(j["l1"], source=src)
 
 
 def pair_count(case) -> int:
@@ -203,11 +210,12 @@ def _pct(xs, q):
     return float(np.percentile(np.asarray(xs, float), q)) if len(xs) else None
 
 
-def ensemble_section(nruns: int, steps: int, warm: int, seed: int = 0) -> dict:
+def ensemble_section(nruns: int, steps: int, warm: int, seed: int = 0, plain_loop: bool = True) -> dict:
     """256 independent 3D columns (npm log-uniform 2k..60k, aspect U[0.5,4],
     phi U[20,40] deg, c U[0,5] kPa): all runs in one engine (one launch per
     stage, graph-replayed) against the plain loop (the same engine code, one
-    run at a time).  Bounded sample: `steps` timed steps per run."""
+    run at a time; skipped without plain_loop).  Bounded sample: `steps` timed
+    steps per run."""
     import torch
 
     from loquat_amd import cases
@@ -230,6 +238,10 @@ def ensemble_section(nruns: int, steps: int, warm: int, seed: int = 0) -> dict:
     failed = sum(k < steps + warm for k in done)
     eng.close()
     torch.cuda.empty_cache()
+    if not plain_loop:
+        return {"workload": f"{nruns} independent 3D column collapses (sum npm {npm_tot})",
+                "steps": steps, "warmup": warm, "batched_M_particle_steps_s": round(work / t_batch / 1e6, 3),
+                "ms_per_step": round(1e3 * t_batch / steps, 4), "failed_runs": failed}
     # plain loop: one run at a time, each a batch of one
     big = max(runs, key=lambda c: c.np)
     eng = Engine(big.np, max(c.derived().ncell for c in runs), 1)
@@ -304,6 +316,7 @@ def served_section(nreq: int, step_scale: float, seed: int = 0) -> dict:
         for rid in finished:
             done_at[ids[rid]] = time.perf_counter() - t0
             fin[ids[rid]] = srv.info(rid)
+            log(f"served: {len(done_at)}/{nreq} done at {done_at[ids[rid]]:.1f} s")
     makespan = time.perf_counter() - t0
     srv.close()
     torch.cuda.empty_cache()
@@ -318,6 +331,7 @@ def served_section(nreq: int, step_scale: float, seed: int = 0) -> dict:
     big = max(reqs, key=lambda r: r[1].np)[1]
     eng = Engine(big.np, max(c.derived().ncell for _, c, _ in reqs), 1)
     clock, lat_plain, busy_plain, work_plain = 0.0, [], 0.0, 0
+    by_size = {"over_200k": [0, 0.0], "under_200k": [0, 0.0]}  # plain loop: work, time per size class
     for a, c, s in reqs:
         t1 = time.perf_counter()
         rid = eng.add(c, s)
@@ -328,8 +342,12 @@ def served_section(nreq: int, step_scale: float, seed: int = 0) -> dict:
         dt = time.perf_counter() - t1
         work_plain += c.npm * steps_done
         busy_plain += dt
+        cls = by_size["over_200k" if c.npm >= 2e5 else "under_200k"]
+        cls[0] += c.npm * steps_done
+        cls[1] += dt
         clock = max(clock, a) + dt
         lat_plain.append(clock - a)
+        log(f"plain loop: {len(lat_plain)}/{nreq} ({c.npm} material particles, {s} steps) in {dt:.2f} s")
     eng.close()
     served = work / busy / 1e6
     plain = work_plain / busy_plain / 1e6
@@ -346,6 +364,7 @@ def served_section(nreq: int, step_scale: float, seed: int = 0) -> dict:
                                  "ticks with runs resident; plain loop: add + steps + remove of each run)",
         "makespan_M_particle_steps_s": round(work / makespan / 1e6, 3),
         "work_share_runs_over_200k": round(big_work / max(work, 1), 4),
+        "plain_loop_M_particle_steps_s_by_size": {k: round(w / max(t, 1e-9) / 1e6, 3) for k, (w, t) in by_size.items()},
         "np_per_npm": round(sum(c.np for _, c, _ in reqs) / sum(c.npm for _, c, _ in reqs), 3),
         "failed_runs": failed,
         "makespan_s": round(makespan, 3),
@@ -412,6 +431,8 @@ def main() -> None:
     ap.add_argument("--no-extra", action="store_true", help="skip the ensemble / served / slope sections")
     ap.add_argument("--slope-only", action="store_true",
                     help="time only the embankment section and print its JSON (for profiling that workload)")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only the batched ensemble (no plain loop): --ensemble-steps timed after --warmup")
     ap.add_argument("--served-only", action="store_true",
                     help="run only the served section (with --serve-requests / --serve-step-scale) and print its JSON")
     ap.add_argument("--cell-order", type=int, default=4, metavar="B",
@@ -441,6 +462,10 @@ def main() -> None:
     if a.slope_only:
         sec = slope_section(a.slope_steps, autotune=bool(a.autotune), cell_order=a.cell_order)
         print(json.dumps({"slope_vw": sec}), flush=True)
+        return
+    if a.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_section(a.ensemble_runs, a.ensemble_steps, a.warmup,
+                                                       plain_loop=False)}), flush=True)
         return
     if a.served_only:
         print(json.dumps({"served": served_section(a.serve_requests, a.serve_step_scale)}), flush=True)
@@ -582,6 +607,7 @@ def main() -> None:
             "bytes_per_launch": bytes_launch,
             "hbm": {"achieved_gbs": round(gbs, 2), "peak_gbs": HBM_PEAK_GBS,
                     "frac": round(gbs / HBM_PEAK_GBS, 5)},
+            "l1": measured_l1(roofline_kernel(variant)),
         },
         "stage_ms_per_step": {k: round(v, 4) for k, v in
                               zip(("grid", "adami", "interaction", "integrate", "shepard"), stage)},
