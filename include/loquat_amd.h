@@ -388,9 +388,32 @@ lq_status lq_engine_set_cell_order(lq_engine* eng, int32_t zbin_bits);
 /* Neighbour count of every particle of a run, in the run's input order:
  * material particles get the number of particles (walls and material, not
  * itself) the interaction pairs them with on the resident state, walls 0.
- * Sorts the resident state first (as a step does).  counts: host, np ints.
- * Diagnostic; the count semantics are lq_particle_interaction's. */
+ * Sorts the resident state first (every run afresh).  counts: host, np
+ * ints.  Diagnostic; the count semantics are lq_particle_interaction's. */
 lq_status lq_engine_neighbour_counts(lq_engine* eng, int32_t run_id, int32_t* counts);
+/* The same counts from the cell tables the runs keep between builds, as the
+ * next step's predictor half scans them (no sort; reach widened by each
+ * run's recorded displacement since its table was built): they equal the
+ * fresh table's at any point of the sort interval.  Overwrites the engine's
+ * last interaction rates (the next step recomputes them).  Diagnostic. */
+lq_status lq_engine_neighbour_counts_stale(lq_engine* eng, int32_t run_id, int32_t* counts);
+
+/* Sort interval (engine setting; no counterpart in the reference, which
+ * sorts in both half steps of every step, src/loquat.cu:345-366).  A run's
+ * particles are sorted and its cell table rebuilt at the start of every
+ * `steps`-th step of its own and by each Shepard filter; in between they keep
+ * their order and table, and every scan widens its reach by how far the
+ * run's particles have moved since the table was built (tracked exactly from
+ * each particle's position at the build), so that every neighbour set is
+ * that of a fresh table.  The schedule counts the run's own steps, so a run
+ * gives the same bits alone and in any batch.  A run whose particles move
+ * more than 0.45 cell between builds is stopped as failed.  Only the order of
+ * the FP32 pair sums differs between intervals.  1 .. LQ_MAX_SORT_INTERVAL;
+ * default LQ_DEFAULT_SORT_INTERVAL. */
+#define LQ_DEFAULT_SORT_INTERVAL 4
+#define LQ_MAX_SORT_INTERVAL 16
+lq_status lq_engine_set_sort_interval(lq_engine* eng, int32_t steps);
+int32_t lq_engine_get_sort_interval(const lq_engine* eng);
 
 /* Mean device time per step of each stage over the last lq_engine_profile
  * call: grid (hash+sort+table+permute), adami, interaction, integrate,

@@ -35,6 +35,13 @@ struct RunDesc {
     int32_t nsteps;       // steps requested
     int32_t zbits;        // sort key = cell << zbits | z-bin in cell (0: cell only)
     int32_t until_time;   // 1: the run ends when time >= time_max (dynamic dt), else after nsteps
+    // The run's cell table is rebuilt every sort-interval steps of its own
+    // (engine setting) and after a Shepard filter; in between its particles
+    // keep the sort keys of the last build and its scans read that table.
+    int32_t since_sort;   // steps completed since the run's table was built
+    int32_t sort_due;     // this step's grid build gives the run fresh keys (k_step_begin)
+    float dtab;           // bound on how far a particle of the run has moved since the table was built (last step end)
+    int32_t out_tab;      // some particle of the run was outside the grid at the last step end
     double time;          // FP64 simulated time (src/loquat.cu:305,455)
     lq_vw_params vw;      // constitutive model 3 constants
     float vw_a, vw_fb0;   // derived model-3 constants (a, and the f_b prefactor)
@@ -73,12 +80,15 @@ struct IndexMap {
     __device__ __forceinline__ int operator()(int t) const { return idx ? idx[t] : base + t; }
 };
 
-// A cell table built before the particles last moved ("stale"): the FP64
-// positions of the particles being updated when it was built, and a bound on
-// how far any particle of a run has moved since -- per run as float bits
-// (dmax[run * DMAX_SLOTS]), or one value (disp) when dmax is null.
-// nout (nullable, per run): nonzero when some particle of the run lies
-// outside the grid now (null: assume so).
+// A cell table built before the particles last moved ("stale"): the
+// positions of the particles being updated when it was built -- FP32 (po) or
+// FP64 (xo, yo, zo) -- and a bound on how far any particle of a run has
+// moved since: per run as float bits (dmax[run * DMAX_SLOTS]), or one value
+// (disp) when dmax is null.  nout (nullable, per run): nonzero when some
+// particle of the run lies outside the grid now (null: assume so).
+// from_desc: the bound and the outside flag are the run descriptor's dtab
+// and out_tab (a table from an earlier step); 1: a run whose table was
+// rebuilt this step (sort_due) has bound 0.
 struct StaleIn {
     const double* xo;
     const double* yo;
@@ -86,7 +96,13 @@ struct StaleIn {
     const uint32_t* dmax;
     float disp;
     const uint32_t* nout;
+    const float4* po;
+    int from_desc;
 };
+
+// Which runs a grid build gives fresh keys: all, those with sort_due or
+// shepard_due set, or none (every particle keeps the key of the last build).
+enum GridRekey { REKEY_ALL = 0, REKEY_SORT_DUE = 1, REKEY_SHEPARD_DUE = 2, REKEY_NONE = 3 };
 
 // Per-run displacement maxima are accumulated in DMAX_SLOTS slots per run
 // (workgroup b uses slot b % DMAX_SLOTS, so that atomics do not pile up on
@@ -108,6 +124,7 @@ struct DevState {
     tensor2d* str;
     tensor2d* stn;
     int32_t* orig;
+    float4* xo;  // engine: each particle's FP32 position when its run's table was built (null in module calls)
 };
 
 struct DevIni {

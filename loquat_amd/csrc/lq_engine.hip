@@ -64,6 +64,9 @@ void fill_desc_static(RunDesc& d, const lq_params& par, const lq_vw_params* vw) 
     d.npb = par.npb;
     d.npm = par.npm;
     d.r2_cut = neighbour_r2_cut(par.i_h);
+    // until a step has seen where the particles are, assume some may lie
+    // outside the grid (the kept-table scans then test every edge hit)
+    d.out_tab = 1;
     d.active = 1;
     d.shepard_due = 1;
     d.nsteps = 0x7fffffff;
@@ -400,6 +403,8 @@ struct Buffers {
     uint32_t* dtred;  // per-run |v|max, |a|max of the dynamic time step (2 x DMAX_SLOTS per run, zero between steps)
     uint32_t* dmax;   // per-run largest displacement of the predictor half (float bits, DMAX_SLOTS per run)
     uint32_t* nout;   // per run: some particle ended the predictor half outside the grid
+    uint32_t* dmaxb;  // per run: the corrector's largest distance from the run's cell table (DMAX_SLOTS per run)
+    uint32_t* noutb;  // per run: some particle ended the corrector outside the grid
     int32_t* mat_idx;   // particle index of the t-th material particle (all runs)
     int32_t* wall_idx;  // particle index of the t-th wall particle (all runs)
     GridBuffers grid;
@@ -431,9 +436,11 @@ struct lq_engine {
     int variant = LQ_DEFAULT_INTERACTION_VARIANT;  // interaction variant (set at creation, by the autotuner or
                                                    // lq_engine_set_interaction_variant)
     int zbits = 4;     // z-bins per cell (log2) in the engine's sort key
+    int sort_every = LQ_DEFAULT_SORT_INTERVAL;  // a run's cell table is rebuilt every sort_every of its steps
     bool any_dynamic = false;  // some resident run has dynamic-time-step = 1
     int64_t nmat_art = 0;      // material particles of runs with artificial-pressure = 1 (last in mat_idx)
-    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    // per (buffer, Shepard step, grid build: none / some runs / all runs)
+    hipGraphExec_t graph[2][2][3] = {};
 };
 
 namespace {
@@ -444,10 +451,11 @@ int zbits_eff(const lq_engine* e) { return std::max(0, std::min(e->zbits, 32 - r
 void drop_graphs(lq_engine* e) {
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
-            if (e->graph[i][j]) {
-                (void)hipGraphExecDestroy(e->graph[i][j]);
-                e->graph[i][j] = nullptr;
-            }
+            for (int k = 0; k < 3; ++k)
+                if (e->graph[i][j][k]) {
+                    (void)hipGraphExecDestroy(e->graph[i][j][k]);
+                    e->graph[i][j][k] = nullptr;
+                }
 }
 
 struct Prof {
@@ -474,30 +482,58 @@ struct StageTimer {
 
 enum { PG_GRID = 0, PG_ADAMI = 1, PG_INTER = 2, PG_INTEG = 3, PG_SHEP = 4 };
 
-void enqueue_build(lq_engine* e, const DevState& st, hipStream_t s) {
+// Grid build of the resident state (which: the runs re-keyed, GridRekey) and
+// the permute into the other state buffer; returns that buffer.
+int enqueue_sort(lq_engine* e, int cur, int which, tensor2d* str_dst, hipStream_t s) {
     Buffers& b = e->b;
     const int n = (int)e->np_total;
-    launch_grid_build_table(b.desc, b.run_of, st.pospres, n, e->ncell_total, zbits_eff(e), b.grid, s);
+    launch_grid_build_table(b.desc, b.run_of, b.S[cur].pospres, n, e->ncell_total, zbits_eff(e), b.grid, s, which);
+    launch_permute(b.grid.vals[b.grid.sorted_slot], n, b.S[cur], b.S[1 - cur], nullptr, nullptr, str_dst, s, b.desc,
+                   b.run_of, which);
+    return 1 - cur;
+}
+
+// Sorts every resident run afresh, so that the neighbour table is that of the
+// current positions (the runs' recorded bounds dtab stay: conservative).
+void sort_resident(lq_engine* e) {
+    e->cur = enqueue_sort(e, e->cur, REKEY_ALL, nullptr, e->stream);
+}
+
+// Host mirror of k_step_begin's sort_due.
+bool sort_due(const lq_engine* e, const RunDesc& d) { return d.active && d.since_sort >= e->sort_every; }
+
+// Which grid build a step needs: 0 none (every active run keeps its table),
+// 1 the runs due, 2 all runs.
+int sort_state(const lq_engine* e) {
+    int due = 0, active = 0;
+    for (const RunSlot& r : e->runs) {
+        if (!r.desc.active) continue;
+        ++active;
+        due += sort_due(e, r.desc) ? 1 : 0;
+    }
+    return due == 0 ? 0 : (due == active ? 2 : 1);
 }
 
 // One full step for every resident run, starting from buffer `cur`; returns
 // the buffer that holds the state afterwards (step_out).
 //
-// The neighbour search runs once per step, in the predictor half (sort,
-// permute, cell table), and again before a Shepard filter.  The corrector
-// half scans the predictor half's table: particles have moved by at most the
-// displacement the predictor records per run (hdt |v|, a small fraction of a
-// cell), and the stale scans widen their reach by it and test each hit
-// against its current cell, so every neighbour set is the one a fresh sort
-// gives (the reference sorts again, src/loquat.cu:362-366).  The ini set is
-// in the order of the state, so the corrector reads it directly.
-int enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
+// A run's cell table is rebuilt at the start of every sort_every-th step of
+// its own (sort_state: none, some or all of the resident runs), and by a
+// Shepard filter; the runs not due keep their sort keys, so their particles
+// stay in place and their table is the one of the last build.  Between
+// builds every scan reads the kept table: the particles have moved by at
+// most the run's recorded bound since (dtab at step start, the predictor's
+// dmax in the corrector half), and the stale scans widen their reach by it
+// and test each hit against its current cell, so every neighbour set is the
+// one a fresh sort gives (the reference sorts in both half steps,
+// src/loquat.cu:345-366).  The ini set is in the order of the state, so the
+// corrector reads it directly.
+int enqueue_step(lq_engine* e, int cur, bool shepard, int sorts, Prof* prof) {
     Buffers& b = e->b;
     hipStream_t s = e->stream;
-    const int n = (int)e->np_total;
     const int nr = (int)e->runs.size();
     const int var = e->variant;
-    launch_step_begin(b.desc, nr, b.dmax, b.nout, s);
+    launch_step_begin(b.desc, nr, b.dmax, b.nout, s, e->sort_every, b.fail);
     if (e->any_dynamic)
         launch_dynamic_dt(b.desc, b.run_of, b.S[cur], b.R, IndexMap{b.mat_idx, 0}, (int)e->nmat_total, nr, b.dtred, s);
     int c = cur;
@@ -505,16 +541,22 @@ int enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
     const int nw = (int)e->nwall_total, nm = (int)e->nmat_total;
     for (int half = 0; half < 2; ++half) {
         const bool corr = half == 1;
-        if (!corr) {
+        if (!corr && sorts > 0) {
             StageTimer t(prof, s, PG_GRID);
-            enqueue_build(e, b.S[c], s);
-            launch_permute(b.grid.vals[b.grid.sorted_slot], n, b.S[c], b.S[1 - c], nullptr, nullptr, nullptr, s);
-            c = 1 - c;
+            c = enqueue_sort(e, c, sorts == 2 ? REKEY_ALL : REKEY_SORT_DUE, nullptr, s);
         }
-        // corrector half: the table and the positions it was built from (the
-        // ini set holds the material particles' positions of that time)
-        const StaleIn stale{b.I.xd, b.I.yd, b.I.zd, b.dmax, 0.0f, b.nout};
-        const StaleIn* sp = corr ? &stale : nullptr;
+        // predictor half: the tables kept from earlier steps (from_desc 1:
+        // the runs re-keyed just now have bound 0; 2: none was); corrector
+        // half: the table of this step's start and the predictor's bound
+        StaleIn stale{};
+        stale.po = b.S[c].xo;
+        if (corr) {
+            stale.dmax = b.dmax;
+            stale.nout = b.nout;
+        } else {
+            stale.from_desc = sorts == 1 ? 1 : 2;
+        }
+        const StaleIn* sp = (corr || sorts != 2) ? &stale : nullptr;
         {
             StageTimer t(prof, s, PG_ADAMI);
             launch_adami_table(b.desc, b.run_of, b.S[c], b.grid.cell_beg, b.grid.cell_end, walls, nw, var, sp, s);
@@ -526,28 +568,27 @@ int enqueue_step(lq_engine* e, int cur, bool shepard, Prof* prof) {
         }
         {
             StageTimer t(prof, s, PG_INTEG);
-            launch_integrate_table(corr, b.desc, b.run_of, b.S[c], b.I, b.R, mats, nm, b.fail, b.dmax, b.nout, s);
+            launch_integrate_table(corr, b.desc, b.run_of, b.S[c], b.I, b.R, mats, nm, b.fail, b.dmax, b.nout, s,
+                                   b.dmaxb, b.noutb);
             if (!corr) launch_dmax_fold(b.dmax, nr, s);
         }
     }
     if (shepard) {
         {
             StageTimer t(prof, s, PG_GRID);
-            enqueue_build(e, b.S[c], s);
-            launch_permute(b.grid.vals[b.grid.sorted_slot], n, b.S[c], b.S[1 - c], nullptr, nullptr, b.T, s);
+            c = enqueue_sort(e, c, REKEY_SHEPARD_DUE, b.T, s);
         }
-        c = 1 - c;
         StageTimer t(prof, s, PG_SHEP);
-        launch_shepard_table(b.desc, b.run_of, b.S[c], b.T, b.S[c].str, b.grid.cell_beg, b.grid.cell_end, n, var,
-                             s);
+        launch_shepard_table(b.desc, b.run_of, b.S[c], b.T, b.S[c].str, b.grid.cell_beg, b.grid.cell_end,
+                             (int)e->np_total, var, s);
     }
-    launch_step_end(b.desc, nr, b.fail, s);
+    launch_step_end(b.desc, nr, b.fail, s, b.dmaxb, b.noutb, sorts > 0, shepard);
     return c;
 }
 
-// Buffer holding the state after a step from `cur` (one permute, plus one
-// for a Shepard filter).
-inline int step_out(int cur, bool shepard) { return shepard ? cur : 1 - cur; }
+// Buffer holding the state after a step from `cur` (one permute per grid
+// build, and one for a Shepard filter).
+inline int step_out(int cur, bool shepard, int sorts) { return (shepard != (sorts > 0)) ? 1 - cur : cur; }
 
 // Host mirror of k_step_begin's Shepard test over all active runs.
 bool any_shepard_due(const lq_engine* e) {
@@ -559,10 +600,15 @@ bool any_shepard_due(const lq_engine* e) {
     return false;
 }
 
-void host_step_end(lq_engine* e) {
+// Host mirror of k_step_end (sorts: the step's grid build, shep: whether it
+// ran a Shepard filter).
+void host_step_end(lq_engine* e, int sorts, bool shep) {
     for (RunSlot& r : e->runs) {
         RunDesc& d = r.desc;
         if (!d.active) continue;
+        const int f = d.par.stress_shepard_filter;
+        const bool shep_sorted = shep && f != 0 && d.step % f == 0 && d.step != 0;
+        d.since_sort = shep_sorted ? 0 : ((sorts > 0 && sort_due(e, d)) ? 1 : d.since_sort + 1);
         d.time += (double)d.par.dt;
         d.step += 1;
         if (d.step >= d.nsteps) d.active = 0;
@@ -589,6 +635,10 @@ lq_status sync_descs(lq_engine* e) {
         d.par.dt = h[i].par.dt;
         d.active = h[i].active;
         d.shepard_due = h[i].shepard_due;
+        d.since_sort = h[i].since_sort;
+        d.sort_due = h[i].sort_due;
+        d.dtab = h[i].dtab;
+        d.out_tab = h[i].out_tab;
         // the corrector's failure flag, before any upload writes the host
         // copy back (lq_engine_poll reads it the same way)
         if (f[i] && !e->runs[i].failed) {
@@ -639,6 +689,9 @@ lq_status relayout(lq_engine* e) {
     LQ_HIP_TRY(hipStreamSynchronize(s));
     const DevState& src = b.S[e->cur];
     const DevState& dst = b.S[1 - e->cur];
+    // the sort keys each run keeps move with its segment and cell range
+    const int ks = b.grid.sorted_slot;
+    const int zb = zbits_eff(e);
     int64_t off = 0, cbase = 0;
     std::vector<int32_t> run_of;
     for (size_t i = 0; i < e->runs.size(); ++i) {
@@ -658,6 +711,10 @@ lq_status relayout(lq_engine* e) {
             LQ_HIP_TRY(cp(dst.str, src.str, 24));
             LQ_HIP_TRY(cp(dst.stn, src.stn, 24));
             LQ_HIP_TRY(cp(dst.orig, src.orig, 4));
+            LQ_HIP_TRY(cp(dst.xo, src.xo, 16));
+            if (ks >= 0)
+                launch_shift_keys(b.grid.keys[ks] + o, b.grid.keys[1 - ks] + off, (int)np,
+                                  (uint32_t)((cbase - (int64_t)d.cell_base) * ((int64_t)1 << zb)), s);
         }
         d.p_off = (int32_t)off;
         d.cell_base = (int32_t)cbase;
@@ -672,6 +729,15 @@ lq_status relayout(lq_engine* e) {
         LQ_HIP_TRY(hipMemcpyAsync(b.run_of, run_of.data(), run_of.size() * 4, hipMemcpyHostToDevice, s));
     LQ_TRY(upload_descs(e));
     drop_graphs(e);
+    // The cell table of the new layout: every run keeps its keys (moved
+    // above), unless none were kept or their format changed (the z-bin bits
+    // depend on the total cell count): then every run is sorted afresh.
+    if (e->np_total > 0) {
+        const bool keep = ks >= 0 && zbits_eff(e) == zb;
+        if (keep) b.grid.sorted_slot = 1 - ks;
+        e->cur = enqueue_sort(e, e->cur, keep ? REKEY_NONE : REKEY_ALL, nullptr, s);
+        LQ_TRY(last_launch("relayout"));
+    }
     return LQ_OK;
 }
 
@@ -712,7 +778,7 @@ extern "C" lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t
              dmalloc(S.zd, np_cap, e->owned) && dmalloc(S.pospres, np_cap + SCAN_PAD, e->owned) &&
              dmalloc(S.velrhop, np_cap, e->owned) && dmalloc(S.idep, np_cap, e->owned) &&
              dmalloc(S.str, np_cap, e->owned) && dmalloc(S.stn, np_cap, e->owned) &&
-             dmalloc(S.orig, np_cap, e->owned);
+             dmalloc(S.orig, np_cap, e->owned) && dmalloc(S.xo, np_cap, e->owned);
     }
     {
         DevIni& I = b.I;
@@ -725,8 +791,11 @@ extern "C" lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t
          dmalloc(b.run_of, np_cap, e->owned) && dmalloc(b.desc, max_runs, e->owned) &&
          dmalloc(b.fail, max_runs, e->owned) && dmalloc(b.mat_idx, np_cap, e->owned) &&
          dmalloc(b.wall_idx, np_cap, e->owned) && dmalloc(b.dtred, 2 * (int64_t)max_runs * DMAX_SLOTS, e->owned) &&
-         dmalloc(b.dmax, (int64_t)max_runs * DMAX_SLOTS, e->owned) && dmalloc(b.nout, max_runs, e->owned);
-    ok = ok && hipMemset(b.dtred, 0, 2 * (size_t)max_runs * DMAX_SLOTS * sizeof(uint32_t)) == hipSuccess;
+         dmalloc(b.dmax, (int64_t)max_runs * DMAX_SLOTS, e->owned) && dmalloc(b.nout, max_runs, e->owned) &&
+         dmalloc(b.dmaxb, (int64_t)max_runs * DMAX_SLOTS, e->owned) && dmalloc(b.noutb, max_runs, e->owned);
+    ok = ok && hipMemset(b.dtred, 0, 2 * (size_t)max_runs * DMAX_SLOTS * sizeof(uint32_t)) == hipSuccess &&
+         hipMemset(b.dmaxb, 0, (size_t)max_runs * DMAX_SLOTS * sizeof(uint32_t)) == hipSuccess &&
+         hipMemset(b.noutb, 0, (size_t)max_runs * sizeof(uint32_t)) == hipSuccess;
     if (!ok || alloc_grid(b.grid, np_cap, ncell_cap) != LQ_OK) {
         for (void* p : e->owned) (void)hipFree(p);
         (void)hipStreamDestroy(e->stream);
@@ -773,6 +842,19 @@ extern "C" lq_status lq_engine_set_interaction_variant(lq_engine* e, int32_t var
 
 extern "C" int32_t lq_engine_get_interaction_variant(const lq_engine* e) { return e ? e->variant : -1; }
 
+extern "C" lq_status lq_engine_set_sort_interval(lq_engine* e, int32_t steps) {
+    if (!e || steps < 1 || steps > LQ_MAX_SORT_INTERVAL) {
+        set_error("lq_engine_set_sort_interval: steps must be 1 .. LQ_MAX_SORT_INTERVAL");
+        return LQ_ERR_INVALID_ARG;
+    }
+    LQ_HIP_TRY(hipStreamSynchronize(e->stream));
+    e->sort_every = steps;
+    drop_graphs(e);
+    return LQ_OK;
+}
+
+extern "C" int32_t lq_engine_get_sort_interval(const lq_engine* e) { return e ? e->sort_every : -1; }
+
 extern "C" lq_status lq_engine_set_cell_order(lq_engine* e, int32_t zbin_bits) {
     if (!e || zbin_bits < 0 || zbin_bits > 8) {
         set_error("lq_engine_set_cell_order: zbin_bits must be 0..8");
@@ -782,7 +864,9 @@ extern "C" lq_status lq_engine_set_cell_order(lq_engine* e, int32_t zbin_bits) {
     e->zbits = zbin_bits;
     LQ_TRY(upload_descs(e));
     drop_graphs(e);
-    return LQ_OK;
+    // the kept sort keys carry the z-bins: every run is sorted afresh
+    sort_resident(e);
+    return last_launch("lq_engine_set_cell_order");
 }
 
 static int64_t steps_until(double t, double tmax, double dt) {
@@ -898,19 +982,23 @@ extern "C" lq_status lq_engine_add_run(lq_engine* e, const lq_run_input* in, int
         LQ_HIP_TRY(hipMemcpy(S.orig + o, orig.data(), np * 4, hipMemcpyHostToDevice));
         LQ_HIP_TRY(hipMemcpy(e->b.run_of + o, run_of.data(), np * 4, hipMemcpyHostToDevice));
     }
+    // Initial neighbour search + Primary sort (src/loquat.cu:297-298) of the
+    // new run only: the resident runs keep their keys and tables (their
+    // segments do not move), unless the key format changes with the larger
+    // cell count.
+    const int zb_before = zbits_eff(e);
+    const bool had_keys = !e->runs.empty() && e->b.grid.sorted_slot >= 0;
+    for (RunSlot& r : e->runs) r.desc.sort_due = 0;
+    slot.desc.sort_due = 1;
+    slot.desc.since_sort = 0;
     e->runs.push_back(slot);
     e->np_total += np;
     e->ncell_total += par.ncell;
     LQ_TRY(upload_descs(e));
     drop_graphs(e);
-    // Initial neighbour search + Primary sort (src/loquat.cu:297-298).  For
-    // the runs already resident this re-sorts by unchanged keys, which the
-    // next step's stable sort would do anyway.
-    Buffers& b = e->b;
-    enqueue_build(e, b.S[e->cur], e->stream);
-    launch_permute(b.grid.vals[b.grid.sorted_slot], (int)e->np_total, b.S[e->cur], b.S[1 - e->cur], nullptr, nullptr,
-                   nullptr, e->stream);
-    e->cur = 1 - e->cur;
+    const bool keep = had_keys && zbits_eff(e) == zb_before;
+    e->cur = enqueue_sort(e, e->cur, keep ? REKEY_SORT_DUE : REKEY_ALL, nullptr, e->stream);
+    e->runs.back().desc.sort_due = 0;
     LQ_TRY(last_launch("lq_engine_add_run"));
     LQ_HIP_TRY(hipStreamSynchronize(e->stream));
     *run_id = slot.id;
@@ -948,12 +1036,13 @@ extern "C" lq_status lq_engine_step(lq_engine* e, int32_t nsteps) {
         for (const RunSlot& r : e->runs) any_active = any_active || r.desc.active;
         if (!any_active) break;
         const bool shep = any_shepard_due(e);
+        const int sorts = sort_state(e);
         if (e->use_graphs) {
-            hipGraphExec_t& g = e->graph[e->cur][shep ? 1 : 0];
+            hipGraphExec_t& g = e->graph[e->cur][shep ? 1 : 0][sorts];
             if (!g) {
                 hipGraph_t graph = nullptr;
                 LQ_HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-                (void)enqueue_step(e, e->cur, shep, nullptr);
+                (void)enqueue_step(e, e->cur, shep, sorts, nullptr);
                 LQ_HIP_TRY(hipStreamEndCapture(e->stream, &graph));
                 hipError_t err = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(graph);
@@ -965,24 +1054,15 @@ extern "C" lq_status lq_engine_step(lq_engine* e, int32_t nsteps) {
             }
             LQ_HIP_TRY(hipGraphLaunch(g, e->stream));
         } else {
-            (void)enqueue_step(e, e->cur, shep, nullptr);
+            (void)enqueue_step(e, e->cur, shep, sorts, nullptr);
             LQ_TRY(last_launch("lq_engine_step"));
         }
-        e->cur = step_out(e->cur, shep);
-        host_step_end(e);
+        e->cur = step_out(e->cur, shep, sorts);
+        host_step_end(e, sorts, shep);
     }
     return LQ_OK;
 }
 
-// Sorts the resident state as a step would (the next step's stable sort by
-// the same keys leaves it unchanged), so that the neighbour table is current.
-static void sort_resident(lq_engine* e) {
-    Buffers& b = e->b;
-    enqueue_build(e, b.S[e->cur], e->stream);
-    launch_permute(b.grid.vals[b.grid.sorted_slot], (int)e->np_total, b.S[e->cur], b.S[1 - e->cur], nullptr, nullptr,
-                   nullptr, e->stream);
-    e->cur = 1 - e->cur;
-}
 
 extern "C" lq_status lq_engine_neighbour_counts(lq_engine* e, int32_t run_id, int32_t* counts) {
     if (!e || !counts) {
@@ -1018,6 +1098,56 @@ extern "C" lq_status lq_engine_neighbour_counts(lq_engine* e, int32_t run_id, in
     (void)hipFree(nbr);
     if (err != hipSuccess) {
         set_error(std::string("lq_engine_neighbour_counts: ") + hipGetErrorString(err));
+        return LQ_ERR_HIP;
+    }
+    for (int k = 0; k < d.np; ++k) {
+        const int t = orig[k];
+        if (t < 0 || t >= d.np) {
+            set_error("corrupt original-index array");
+            return LQ_ERR_STATE;
+        }
+        counts[t] = k < d.npb ? 0 : c[k];
+    }
+    return LQ_OK;
+}
+
+extern "C" lq_status lq_engine_neighbour_counts_stale(lq_engine* e, int32_t run_id, int32_t* counts) {
+    if (!e || !counts) {
+        set_error("lq_engine_neighbour_counts_stale: null argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    const int i = find_run(e, run_id);
+    if (i < 0) {
+        set_error("unknown run id");
+        return LQ_ERR_INVALID_ARG;
+    }
+    Buffers& b = e->b;
+    hipStream_t s = e->stream;
+    LQ_HIP_TRY(hipStreamSynchronize(s));
+    int32_t* nbr = nullptr;
+    if (hipMalloc(&nbr, (size_t)std::max<int64_t>(e->np_total, 1) * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("lq_engine_neighbour_counts_stale: device allocation failed");
+        return LQ_ERR_OUT_OF_MEMORY;
+    }
+    const IndexMap mats{b.mat_idx, 0};
+    // the scan of a step's predictor half on the tables the runs keep
+    StaleIn stale{};
+    stale.po = b.S[e->cur].xo;
+    stale.from_desc = 2;
+    const int var = e->variant;
+    launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
+                             (int)e->nmat_total, (int)e->nmat_art, nbr, var, &stale, s);
+    const RunDesc& d = e->runs[i].desc;
+    std::vector<int32_t> c(d.np), orig(d.np);
+    hipError_t err = hipGetLastError();
+    if (err == hipSuccess && d.np > 0) err = hipMemcpyAsync(c.data(), nbr + d.p_off, d.np * 4, hipMemcpyDeviceToHost, s);
+    if (err == hipSuccess && d.np > 0)
+        err = hipMemcpyAsync(orig.data(), b.S[e->cur].orig + d.p_off, d.np * 4, hipMemcpyDeviceToHost, s);
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
+    (void)hipFree(nbr);
+    if (err != hipSuccess) {
+        set_error(std::string("lq_engine_neighbour_counts_stale: ") + hipGetErrorString(err));
         return LQ_ERR_HIP;
     }
     for (int k = 0; k < d.np; ++k) {
@@ -1165,8 +1295,9 @@ extern "C" lq_status lq_engine_profile(lq_engine* e, int32_t nsteps, double* ms_
         for (const RunSlot& r : e->runs) any_active = any_active || r.desc.active;
         if (!any_active) break;
         const bool shep = any_shepard_due(e);
-        e->cur = enqueue_step(e, e->cur, shep, &p);
-        host_step_end(e);
+        const int sorts = sort_state(e);
+        e->cur = enqueue_step(e, e->cur, shep, sorts, &p);
+        host_step_end(e, sorts, shep);
         ++done;
     }
     (void)hipEventDestroy(p.ev[0]);

@@ -190,15 +190,29 @@ __device__ __forceinline__ uint32_t cell_key(const DP& dp, int k, float4 p) {
            (d.zbits ? zbin_in_cell(p.z, i_h, d.zbits, cz) : 0u);
 }
 
+// Whether a build re-keys particle k's run (`which`, see GridRekey); the
+// others keep the key of the last build, kept[k].
+template <class DP>
+__device__ __forceinline__ bool rekeyed(const DP& dp, int k, int which) {
+    if (which == REKEY_ALL) return true;
+    if (which == REKEY_NONE) return false;
+    const RunDesc& d = dp.get(dp.run(k));
+    return which == REKEY_SORT_DUE ? d.sort_due != 0 : d.shepard_due != 0;
+}
+
 // Hash fused with the first radix pass's count: one workgroup per tile
 // computes the keys of its tile (stored for the scatter), counts their low
 // digit as k_radix_count does, and clears its share of the cell table
 // (cell_beg = cell_end = -1; the occupied cells are written after the sort).
+// Particles of runs that `which` does not re-key take their key from kept
+// (the sorted keys of the last build, in the current particle order): their
+// stable order and their cells are then those of the last build.
 template <class DP>
 __global__ __launch_bounds__(RS_THREADS) void k_hash_count(DP dp, const float4* __restrict__ pospres, int n,
                                                             uint32_t* __restrict__ keys, uint32_t* __restrict__ hist,
                                                             int ntiles, int32_t* __restrict__ cbeg,
-                                                            int32_t* __restrict__ cend, int64_t ncell) {
+                                                            int32_t* __restrict__ cend, int64_t ncell,
+                                                            const uint32_t* kept, int which) {
     __shared__ uint32_t cnt[4][RS_BINS];
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     for (int i = t; i < 4 * RS_BINS; i += RS_THREADS) (&cnt[0][0])[i] = 0;
@@ -219,7 +233,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_hash_count(DP dp, const float4* 
         const bool valid = idx < n;
         uint32_t key = 0u;
         if (valid) {
-            key = cell_key(dp, idx, pospres[idx]);
+            // kept may alias keys (the same slot): each thread reads its own
+            // entry before writing it
+            key = (kept && !rekeyed(dp, idx, which)) ? kept[idx] : cell_key(dp, idx, pospres[idx]);
             keys[idx] = key;
         }
         const uint32_t d = key & 255u;
@@ -248,11 +264,15 @@ __global__ void k_cell_table(const uint32_t* __restrict__ keys, int n, int zbits
     if (k == n - 1 || (keys[k + 1] >> zbits) != cell) end[cell] = k + 1;
 }
 
+// With table positions (engine states): particles of re-keyed runs record
+// the position their keys came from; the others keep theirs.
 __global__ __launch_bounds__(256) void k_permute(const uint32_t* __restrict__ perm, int n, DevState src, DevState dst,
-                                                 int with_ini, DevIni isrc, DevIni idst, tensor2d* str_dst) {
+                                                 int with_ini, DevIni isrc, DevIni idst, tensor2d* str_dst,
+                                                 DescTable dp, int which) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int s = (int)perm[k];
+    if (dst.xo) dst.xo[k] = rekeyed(dp, s, which) ? src.pospres[s] : src.xo[s];
     dst.xd[k] = src.xd[s];
     dst.yd[k] = src.yd[s];
     dst.zd[k] = src.zd[s];
@@ -341,7 +361,7 @@ static void radix_passes(GridBuffers& g, int n, int bits, hipStream_t s) {
 
 template <class DP>
 static void grid_build(const DP& dp, const float4* pospres, int n, int bits, int64_t ncell_total, int zbits,
-                       GridBuffers& g, hipStream_t s) {
+                       GridBuffers& g, hipStream_t s, int which = REKEY_ALL) {
     if (n <= 0) {
         g.sorted_slot = 0;
         const int fill_blocks = (int)std::min<int64_t>(blocks_for(ncell_total, 256), 4096);
@@ -349,8 +369,9 @@ static void grid_build(const DP& dp, const float4* pospres, int n, int bits, int
         return;
     }
     const int ntiles = rs_tiles(n);
+    const uint32_t* kept = which != REKEY_ALL && g.sorted_slot >= 0 ? g.keys[g.sorted_slot] : nullptr;
     k_hash_count<DP><<<ntiles, RS_THREADS, 0, s>>>(dp, pospres, n, g.keys[0], g.hist, ntiles, g.cell_beg,
-                                                   g.cell_end, ncell_total);
+                                                   g.cell_end, ncell_total, kept, which);
     radix_passes(g, n, bits, s);
     k_cell_table<<<blocks_for(n, 256), 256, 0, s>>>(g.keys[g.sorted_slot], n, zbits, g.cell_beg, g.cell_end);
 }
@@ -360,8 +381,19 @@ void launch_grid_build_one(const RunDesc& d, const float4* pospres, GridBuffers&
 }
 
 void launch_grid_build_table(const RunDesc* d, const int32_t* run_of, const float4* pospres, int n,
-                             int64_t ncell_total, int zbits, GridBuffers& g, hipStream_t s) {
-    grid_build(DescTable{d, run_of}, pospres, n, rs_bits(ncell_total) + zbits, ncell_total, zbits, g, s);
+                             int64_t ncell_total, int zbits, GridBuffers& g, hipStream_t s, int which) {
+    grid_build(DescTable{d, run_of}, pospres, n, rs_bits(ncell_total) + zbits, ncell_total, zbits, g, s, which);
+}
+
+// dst[i] = src[i] + shift for i < n (a run's kept keys moved to its new
+// segment and cell range).
+__global__ void k_shift_keys(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int n, uint32_t shift) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] + shift;
+}
+
+void launch_shift_keys(const uint32_t* src, uint32_t* dst, int n, uint32_t shift, hipStream_t s) {
+    if (n > 0) k_shift_keys<<<blocks_for(n, 256), 256, 0, s>>>(src, dst, n, shift);
 }
 
 // ------------------------------------------------- grid build without allocation
@@ -431,12 +463,14 @@ lq_status launch_grid_build_noalloc(const RunDesc& d, const float4* pospres, int
 }
 
 void launch_permute(const uint32_t* perm, int n, const DevState& src, const DevState& dst, const DevIni* ini_src,
-                    const DevIni* ini_dst, tensor2d* str_dst, hipStream_t s) {
+                    const DevIni* ini_dst, tensor2d* str_dst, hipStream_t s, const RunDesc* desc,
+                    const int32_t* run_of, int which) {
     if (n <= 0) return;
     const int with_ini = (ini_src && ini_dst) ? 1 : 0;
     DevIni a = with_ini ? *ini_src : DevIni{};
     DevIni b = with_ini ? *ini_dst : DevIni{};
-    k_permute<<<blocks_for(n, 256), 256, 0, s>>>(perm, n, src, dst, with_ini, a, b, str_dst);
+    k_permute<<<blocks_for(n, 256), 256, 0, s>>>(perm, n, src, dst, with_ini, a, b, str_dst,
+                                                 DescTable{desc, run_of}, which);
 }
 
 }  // namespace lq

@@ -65,8 +65,13 @@ void free_grid(GridBuffers& g);
 // (key, index) over ceil(bits/8) 8-bit passes (result in slot
 // g.sorted_slot), and cell_beg / cell_end (-1 for empty cells).
 void launch_grid_build_one(const RunDesc& d, const float4* pospres, GridBuffers& g, hipStream_t s);
+// which (GridRekey): the runs whose particles get fresh keys; the others keep
+// the keys of the last build (g.keys[g.sorted_slot], in the current order).
 void launch_grid_build_table(const RunDesc* d, const int32_t* run_of, const float4* pospres, int n,
-                             int64_t ncell_total, int zbits, GridBuffers& g, hipStream_t s);
+                             int64_t ncell_total, int zbits, GridBuffers& g, hipStream_t s,
+                             int which = REKEY_ALL);
+// dst[i] = src[i] + shift, i < n: kept keys moved with their run's segment.
+void launch_shift_keys(const uint32_t* src, uint32_t* dst, int n, uint32_t shift, hipStream_t s);
 // The grid build of one run into caller-owned arrays (part_cel, part_idx:
 // np entries; cell_beg, cell_end: ncell), the sort's scratch in a
 // file-scope device array of lq_grid.hip (np <= NOALLOC_MAX_PARTICLES).  No
@@ -76,8 +81,11 @@ lq_status launch_grid_build_noalloc(const RunDesc& d, const float4* pospres, int
                                     int32_t* cell_beg, int32_t* cell_end, hipStream_t s);
 // dst[k] = src[perm[k]] for all state arrays (and ini when both set).  When
 // str_dst is non-null the permuted stress goes there instead of dst.str.
+// Engine states also carry table positions: those of the runs `which`
+// re-keyed (desc / run_of) become the positions just sorted.
 void launch_permute(const uint32_t* perm, int n, const DevState& src, const DevState& dst, const DevIni* ini_src,
-                    const DevIni* ini_dst, tensor2d* str_dst, hipStream_t s);
+                    const DevIni* ini_dst, tensor2d* str_dst, hipStream_t s, const RunDesc* desc = nullptr,
+                    const int32_t* run_of = nullptr, int which = REKEY_ALL);
 
 // ---------------------------------------------------------------- physics
 enum Stage { ST_ADAMI = 0, ST_INTERACT = 1, ST_PREDICT = 2, ST_CORRECT = 3, ST_SHEPARD = 4 };
@@ -112,9 +120,12 @@ void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, 
 // displacement (and nout, per run, whether a particle ended outside the
 // grid); the corrector raises fail when the displacement exceeded what a
 // stale table tolerates.
+// dmaxb / noutb (nullable, corrector): the same for where the step leaves the
+// particles relative to their run's cell table (engine states).
 void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
                             const DevIni& ini, const DevRates& rt, IndexMap mats, int nmat, int32_t* fail,
-                            uint32_t* dmax, uint32_t* nout, hipStream_t s);
+                            uint32_t* dmax, uint32_t* nout, hipStream_t s, uint32_t* dmaxb = nullptr,
+                            uint32_t* noutb = nullptr);
 // Shepard filter: reads stress from str_src, writes str_dst (walls and runs
 // not due are copied through).
 void launch_shepard_one(const RunDesc& d, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,
@@ -123,10 +134,16 @@ void launch_shepard_table(const RunDesc* d, const int32_t* run_of, const DevStat
                           tensor2d* str_dst, const int32_t* beg, const int32_t* end, int n, int variant,
                           hipStream_t s);
 // Per-run step bookkeeping on the device.
-void launch_step_begin(RunDesc* d, int nruns, uint32_t* dmax, uint32_t* nout, hipStream_t s);
+// sort_every > 0: sets each run's sort_due for this step, and stops (fail)
+// a run whose kept cell table is further behind than a stale scan covers.
+void launch_step_begin(RunDesc* d, int nruns, uint32_t* dmax, uint32_t* nout, hipStream_t s, int sort_every = 0,
+                       int32_t* fail = nullptr);
 // Folds the predictor's per-run displacement slots (DMAX_SLOTS per run).
 void launch_dmax_fold(uint32_t* dmax, int nruns, hipStream_t s);
-void launch_step_end(RunDesc* d, int nruns, const int32_t* fail, hipStream_t s);
+// dmaxb / noutb (nullable): fold the corrector's bounds into each run's dtab
+// / out_tab and advance since_sort (grid_ran, shep_ran: which builds ran).
+void launch_step_end(RunDesc* d, int nruns, const int32_t* fail, hipStream_t s, uint32_t* dmaxb = nullptr,
+                     uint32_t* noutb = nullptr, bool grid_ran = true, bool shep_ran = false);
 // Dynamic time step of the runs with dynamic-time-step = 1: per-run maxima of
 // |v| (state) and |a| (rates of the previous corrector) into red (2 x
 // DMAX_SLOTS per run, zero on entry, cleared again on exit), then each such

@@ -548,10 +548,18 @@ __device__ __forceinline__ void pair_eval3(const IPart& pi, uint32_t j, const De
     pair_compute(pi, pair_load(st, j), c, a);
 }
 
-__device__ __forceinline__ float stale_disp(const StaleIn& sn, int run) {
+__device__ __forceinline__ float stale_disp(const StaleIn& sn, const RunDesc& d, int run) {
+    if (sn.from_desc) return (sn.from_desc == 1 && d.sort_due) ? 0.0f : d.dtab;
     return sn.dmax ? __uint_as_float(sn.dmax[run * DMAX_SLOTS]) : sn.disp;
 }
-__device__ __forceinline__ bool stale_out_now(const StaleIn& sn, int run) { return !sn.nout || sn.nout[run] != 0u; }
+__device__ __forceinline__ bool stale_out_now(const StaleIn& sn, const RunDesc& d, int run) {
+    if (sn.from_desc) return d.out_tab != 0;
+    return !sn.nout || sn.nout[run] != 0u;
+}
+// Where the table put particle k.
+__device__ __forceinline__ float4 stale_pos(const StaleIn& sn, int k) {
+    return sn.po ? sn.po[k] : make_float4((float)sn.xo[k], (float)sn.yo[k], (float)sn.zo[k], 0.0f);
+}
 
 // Wave-cooperative neighbour scan shared by the list-based kernels.  For a
 // lane with `active` set it walks the z-merged (x, y) columns of cell layers
@@ -942,8 +950,7 @@ k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ be
         ps = st.pospres[k];
         load_part(st, k, pi);
         if constexpr (STALE) {
-            const float4 po = make_float4((float)sn.xo[k], (float)sn.yo[k], (float)sn.zo[k], 0.0f);
-            g = make_geom_stale(d, ps, po, stale_disp(sn, dp.run(k)), stale_out_now(sn, dp.run(k)));
+            g = make_geom_stale(d, ps, stale_pos(sn, k), stale_disp(sn, d, dp.run(k)), stale_out_now(sn, d, dp.run(k)));
         } else {
             g = make_geom(d, ps);
         }
@@ -1021,7 +1028,7 @@ __global__ __launch_bounds__(256) void k_adami_lds(DP dp, DevState st, const int
         yi = st.yd[k];
         zi = st.zd[k];
         // walls do not move: the table's position is the current one
-        g = STALE ? make_geom_stale(d, ps, ps, stale_disp(sn, dp.run(k)), stale_out_now(sn, dp.run(k)))
+        g = STALE ? make_geom_stale(d, ps, ps, stale_disp(sn, d, dp.run(k)), stale_out_now(sn, d, dp.run(k)))
                   : make_geom(d, ps);
     }
     float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
@@ -1289,6 +1296,14 @@ __device__ __forceinline__ void block_max_by_run(int run, float v, uint32_t* red
     }
 }
 
+// Distance of the FP32 position of (x, y, z) from where the cell table put
+// the particle (po), inflated far beyond the rounding of the FP32 arithmetic:
+// the bound the stale scans widen their reach by.
+__device__ __forceinline__ float moved_since_table(float4 po, double x, double y, double z) {
+    const float dx = (float)x - po.x, dy = (float)y - po.y, dz = (float)z - po.z;
+    return sqrtf(dx * dx + dy * dy + dz * dz) * 1.0001f + 1e-12f;
+}
+
 // Predictor (src/predictor_corrector.cu:169-218) fused with the *_ini copy
 // that the reference does as five device-to-device copies (:240-246), for
 // one material particle; returns how far it moved (rounded up).
@@ -1339,6 +1354,7 @@ __device__ __forceinline__ float predict_one(const RunDesc& d, int k, const DevS
     const int cx = cell_coord((float)x, i_h), cy = cell_coord((float)y, i_h), cz = cell_coord((float)z, i_h);
     outside = cx < 0 || cx >= d.par.cell_num_x || cy < 0 || cy >= d.par.cell_num_y || cz < 0 ||
               cz >= d.par.cell_num_z;
+    if (st.xo) return moved_since_table(st.xo[k], x, y, z);
     // |x_new - x_old| = hdt |v|, inflated far beyond the rounding of either
     const float vn = sqrtf(vr.x * vr.x + vr.y * vr.y + vr.z * vr.z);
     return hdt * vn * 1.0001f + 1e-12f;
@@ -1383,16 +1399,11 @@ __global__ void k_dmax_fold(uint32_t* __restrict__ dmax, int nruns) {
 // SortArraysAll of the *_ini arrays, src/loquat.cu:200-213).
 // fail (nullable, one flag per run) is raised when a particle's new state is
 // not finite, so a diverging run can be stopped without touching the others.
-template <class DP>
-__global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni ini, DevRates rt, IndexMap mats,
-                                                   int n, int32_t* __restrict__ fail,
-                                                   const uint32_t* __restrict__ dmax = nullptr) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int k = mats(t);
-    const int run = dp.run(k);
-    const RunDesc& d = dp.get(run);
-    if ((k - d.p_off) < d.npb || !d.active) return;
+// Returns how far the particle is from where its cell table put it (engine
+// states; 0 otherwise) and whether it ended outside the grid.
+__device__ __forceinline__ float correct_one(const RunDesc& d, int run, int k, const DevState& st, const DevIni& ini,
+                                             const DevRates& rt, int32_t* __restrict__ fail,
+                                             const uint32_t* __restrict__ dmax, bool& outside) {
     const float dt = d.par.dt;
     const double dtd = (double)dt;
     const float4 a = rt.accad[k];
@@ -1432,6 +1443,37 @@ __global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni in
     st.idep[k] = idep;
     st.str[k] = s;
     st.stn[k] = stn;
+    const float i_h = d.par.i_h;
+    const int cx = cell_coord((float)x, i_h), cy = cell_coord((float)y, i_h), cz = cell_coord((float)z, i_h);
+    outside = cx < 0 || cx >= d.par.cell_num_x || cy < 0 || cy >= d.par.cell_num_y || cz < 0 ||
+              cz >= d.par.cell_num_z;
+    return st.xo ? moved_since_table(st.xo[k], x, y, z) : 0.0f;
+}
+
+// dmaxb / noutb (nullable, as dmax / nout of k_predictor): where the step
+// leaves each run's particles relative to its cell table, for the next
+// step's scans when the table is kept.
+template <class DP>
+__global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni ini, DevRates rt, IndexMap mats,
+                                                   int n, int32_t* __restrict__ fail,
+                                                   const uint32_t* __restrict__ dmax = nullptr,
+                                                   uint32_t* __restrict__ dmaxb = nullptr,
+                                                   uint32_t* __restrict__ noutb = nullptr) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int run = -1;
+    float disp = 0.0f;
+    if (t < n) {
+        const int k = mats(t);
+        const int r = dp.run(k);
+        const RunDesc& d = dp.get(r);
+        if ((k - d.p_off) >= d.npb && d.active) {
+            run = r;
+            bool outside = false;
+            disp = correct_one(d, r, k, st, ini, rt, fail, dmax, outside);
+            if (outside && noutb) atomicOr(&noutb[r], 1u);
+        }
+    }
+    if (dmaxb) block_max_by_run(run, disp, dmaxb, 1, 0);
 }
 
 // ----------------------------------------------------------------- Shepard
@@ -1538,24 +1580,57 @@ __global__ __launch_bounds__(256) void k_shepard_lds(DP dp, DevState st, const t
     }
 }
 
-__global__ void k_step_begin(RunDesc* d, int nruns, uint32_t* __restrict__ dmax, uint32_t* __restrict__ nout) {
+// sort_every > 0: marks the runs whose cell table is rebuilt this step
+// (sort_due: sort_every steps since the last build), and stops a run whose
+// kept table is further behind its particles than a stale scan covers.
+__global__ void k_step_begin(RunDesc* d, int nruns, uint32_t* __restrict__ dmax, uint32_t* __restrict__ nout,
+                             int sort_every, int32_t* __restrict__ fail) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nruns) return;
     if (nout) nout[r] = 0u;
     if (dmax)
         for (int i = 0; i < DMAX_SLOTS; ++i) dmax[r * DMAX_SLOTS + i] = 0u;
+    if (sort_every > 0) {
+        d[r].sort_due = (d[r].active && d[r].since_sort >= sort_every) ? 1 : 0;
+        if (d[r].active && !d[r].sort_due && !(2.0f * d[r].dtab <= STALE_LIMIT * (2.0f / d[r].par.i_h))) {
+            if (fail) fail[r] = 1;
+            d[r].active = 0;
+        }
+    }
     const int f = d[r].par.stress_shepard_filter;
     const int step = d[r].step;
     d[r].shepard_due = (d[r].active && f != 0 && step % f == 0 && step != 0) ? 1 : 0;
 }
 
-__global__ void k_step_end(RunDesc* d, int nruns, const int32_t* __restrict__ fail) {
+// dmaxb / noutb (nullable): the corrector's bound on how far each run's
+// particles are from its cell table, folded into the run's dtab / out_tab
+// for the next step (and the slots cleared); grid_ran / shep_ran: this step
+// rebuilt the table of the runs with sort_due / shepard_due.
+__global__ void k_step_end(RunDesc* d, int nruns, const int32_t* __restrict__ fail, uint32_t* __restrict__ dmaxb,
+                           uint32_t* __restrict__ noutb, int grid_ran, int shep_ran) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nruns) return;
+    uint32_t moved = 0u, out = 0u;
+    if (dmaxb) {
+        for (int i = 0; i < DMAX_SLOTS; ++i) {
+            moved = max(moved, dmaxb[r * DMAX_SLOTS + i]);
+            dmaxb[r * DMAX_SLOTS + i] = 0u;
+        }
+        out = noutb[r];
+        noutb[r] = 0u;
+    }
     if (!d[r].active) return;
     if (fail && fail[r]) {
         d[r].active = 0;
         return;
+    }
+    if (dmaxb) {
+        // a Shepard filter re-sorted the run after the corrector: its table
+        // is that of the current positions
+        const bool shep_sorted = shep_ran && d[r].shepard_due;
+        d[r].dtab = shep_sorted ? 0.0f : __uint_as_float(moved);
+        d[r].out_tab = out != 0u ? 1 : 0;
+        d[r].since_sort = shep_sorted ? 0 : ((grid_ran && d[r].sort_due) ? 1 : d[r].since_sort + 1);
     }
     d[r].time += (double)d[r].par.dt;
     d[r].step += 1;
@@ -1733,11 +1808,12 @@ void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, 
 }
 void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
                             const DevIni& ini, const DevRates& rt, IndexMap mats, int nmat, int32_t* fail,
-                            uint32_t* dmax, uint32_t* nout, hipStream_t s) {
+                            uint32_t* dmax, uint32_t* nout, hipStream_t s, uint32_t* dmaxb, uint32_t* noutb) {
     if (nmat <= 0) return;
     const int nb = blocks_for(nmat, 256);
     if (corrector)
-        k_corrector<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat, fail, dmax);
+        k_corrector<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat, fail, dmax, dmaxb,
+                                                  noutb);
     else
         k_predictor<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat, dmax, nout);
 }
@@ -1764,16 +1840,20 @@ void launch_shepard_table(const RunDesc* d, const int32_t* run_of, const DevStat
     launch_shepard(DescTable{d, run_of}, st, str_src, str_dst, beg, end, n, variant, s);
 }
 
-void launch_step_begin(RunDesc* d, int nruns, uint32_t* dmax, uint32_t* nout, hipStream_t s) {
-    if (nruns > 0) k_step_begin<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns, dmax, nout);
+void launch_step_begin(RunDesc* d, int nruns, uint32_t* dmax, uint32_t* nout, hipStream_t s, int sort_every,
+                       int32_t* fail) {
+    if (nruns > 0) k_step_begin<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns, dmax, nout, sort_every, fail);
 }
 void launch_dynamic_dt(RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt, IndexMap mats,
                        int nmat, int nruns, uint32_t* red, hipStream_t s) {
     if (nmat > 0) k_dt_reduce<<<blocks_for(nmat, 256), 256, 0, s>>>(d, run_of, st.velrhop, rt.accad, mats, nmat, red);
     if (nruns > 0) k_dt_update<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns, red);
 }
-void launch_step_end(RunDesc* d, int nruns, const int32_t* fail, hipStream_t s) {
-    if (nruns > 0) k_step_end<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns, fail);
+void launch_step_end(RunDesc* d, int nruns, const int32_t* fail, hipStream_t s, uint32_t* dmaxb, uint32_t* noutb,
+                     bool grid_ran, bool shep_ran) {
+    if (nruns > 0)
+        k_step_end<<<blocks_for(nruns, 64), 64, 0, s>>>(d, nruns, fail, dmaxb, noutb, grid_ran ? 1 : 0,
+                                                        shep_ran ? 1 : 0);
 }
 void launch_copy_tensor(const tensor2d* src, tensor2d* dst, int n, hipStream_t s) {
     if (n > 0) k_copy_tensor<<<blocks_for(n, 256), 256, 0, s>>>(src, dst, n);

@@ -269,6 +269,20 @@ class Engine:
         abi.check(abi.lib().lq_engine_neighbour_counts(self.h, rid, out.ctypes.data))
         return out
 
+    def neighbour_counts_stale(self, rid: int) -> np.ndarray:
+        """The same counts from the cell tables the runs keep between sorts
+        (no sort; as the next step's predictor half scans them)."""
+        out = np.zeros(self.info(rid)["np"], np.int32)
+        abi.check(abi.lib().lq_engine_neighbour_counts_stale(self.h, rid, out.ctypes.data))
+        return out
+
+    def set_sort_interval(self, steps: int) -> None:
+        """Sort each run (and rebuild its cell table) every `steps` of its own steps."""
+        abi.check(abi.lib().lq_engine_set_sort_interval(self.h, int(steps)))
+
+    def sort_interval(self) -> int:
+        return abi.lib().lq_engine_get_sort_interval(self.h)
+
     def info(self, rid: int) -> dict:
         s, t, f, n = C.c_int64(), C.c_double(), C.c_int32(), C.c_int32()
         abi.check(abi.lib().lq_engine_run_info(self.h, rid, C.byref(s), C.byref(t), C.byref(f), C.byref(n)))
