@@ -47,14 +47,15 @@ KERNEL_NAMES = {0: "k_interaction_direct", 1: "k_interaction_lds", 2: "k_interac
                 9: "k_interaction_v3<DescTable, 2, true, false, false, 40, 8, 4>",
                 10: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 2>",
                 11: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1>"}
-# The corrector half scans the predictor half's cell table with variant 6's
-# shape, in the workgroup size of variants 10 (128 threads) and 11 (64).
+# Between a run's sorts both half steps scan its kept cell table with variant
+# 6's shape, in the workgroup size of variants 10 (128 threads) and 11 (64);
+# the predictor half of a sort step scans the fresh table with the variant.
 def stale_kernel(variant: int) -> str:
     return f"k_interaction_v3<DescTable, 2, true, false, true, 40, 4, {({10: 2, 11: 1}).get(variant, 4)}>"
 
 
 def roofline_kernel(variant: int) -> str:
-    return f"{KERNEL_NAMES[variant]} (predictor half) + {stale_kernel(variant)} (corrector half)"
+    return f"{stale_kernel(variant)} (kept tables) + {KERNEL_NAMES[variant]} (fresh table at sorts)"
 
 
 def log(msg: str) -> None:
@@ -484,6 +485,7 @@ def main() -> None:
 
     eng = Engine(case.np, p.ncell, 1)
     eng.set_cell_order(a.cell_order)
+    sort_interval = eng.sort_interval()
     rid = eng.add(case)
     tuned = None
     cell_order = a.cell_order
@@ -582,6 +584,7 @@ def main() -> None:
             "npm": case.npm, "np": case.np, "ncell": p.ncell, "dx": 0.005,
             "particle_step": "one material particle advanced one full predictor-corrector step",
             "cell_order_zbits": cell_order,
+            "sort_interval": sort_interval,
             "np_steps_per_s_M": round(case.np * a.steps / (gpu_ms * 1e-3) / 1e6, 3),
             "wall_ms_per_step": round(wall * 1e3 / a.steps, 4),
             "steps_done": info["steps"],

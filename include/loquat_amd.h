@@ -326,10 +326,12 @@ lq_status lq_engine_add_run(lq_engine* eng, const lq_run_input* in, int32_t* run
 lq_status lq_engine_remove_run(lq_engine* eng, int32_t run_id);
 
 /* Advances every unfinished run by up to nsteps steps (runs stop at their own
- * nsteps).  Asynchronous on the engine stream.  Each step searches neighbours
- * once, in the predictor half; the corrector half reuses that cell table
- * with the reach widened by how far the run's particles moved, which gives
- * the neighbour sets of a second search (src/loquat.cu:362-366). */
+ * nsteps).  Asynchronous on the engine stream.  A run's neighbours are
+ * searched afresh (sort and cell table) every sort-interval steps of its own
+ * (lq_engine_set_sort_interval) and by each Shepard filter; every other scan
+ * reuses the run's last table with the reach widened by how far its
+ * particles have moved since, which gives the neighbour sets of a fresh
+ * search (the reference searches in both half steps, src/loquat.cu:345-366). */
 lq_status lq_engine_step(lq_engine* eng, int32_t nsteps);
 lq_status lq_engine_synchronize(lq_engine* eng);
 
@@ -348,11 +350,14 @@ lq_status lq_engine_download(lq_engine* eng, int32_t run_id, lq_host_particles* 
 /* Particle deletion (Parameters::np_delete / ncell_delete, include/types.h:93,108,
  * and the LoquatGrid::ResizeArrayLength hook, src/loquat_grid.h:58, which the
  * reference never calls): removes from every run the material particles that
- * lie outside its cell grid (cell ncell - 1, where the reference's index rule
- * sends them and where no neighbour search finds them), shrinks np and npm,
- * adds them to par.np_delete, and packs the runs again.  The surviving
- * particles' states and order are unchanged, so their results are the same
- * as without compaction.  removed (nullable): particles removed in total. */
+ * lay outside its cell grid when the run was last sorted (cell ncell - 1,
+ * where the reference's index rule sends them and where no neighbour search
+ * finds them; they are the tail of the run's segment), shrinks np and npm,
+ * adds them to par.np_delete, and packs the runs again.  Particles that left
+ * the grid since are removed by a call after the run's next sort.  The
+ * surviving particles' states, order and sort schedule are unchanged, so
+ * their results are the same as without compaction.  removed (nullable):
+ * particles removed in total. */
 lq_status lq_engine_compact(lq_engine* eng, int64_t* removed);
 
 /* Collects the per-run failure flags the corrector raises on the device when
@@ -410,7 +415,7 @@ lq_status lq_engine_neighbour_counts_stale(lq_engine* eng, int32_t run_id, int32
  * more than 0.45 cell between builds is stopped as failed.  Only the order of
  * the FP32 pair sums differs between intervals.  1 .. LQ_MAX_SORT_INTERVAL;
  * default LQ_DEFAULT_SORT_INTERVAL. */
-#define LQ_DEFAULT_SORT_INTERVAL 4
+#define LQ_DEFAULT_SORT_INTERVAL 8
 #define LQ_MAX_SORT_INTERVAL 16
 lq_status lq_engine_set_sort_interval(lq_engine* eng, int32_t steps);
 int32_t lq_engine_get_sort_interval(const lq_engine* eng);

@@ -89,7 +89,7 @@ REQ_STATE = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "rejected"}
 INTERACTION_VARIANTS = 12  # LQ_INTERACTION_VARIANTS
 CELL_ORDER_CANDIDATES = 6  # LQ_CELL_ORDER_CANDIDATES
 DEFAULT_INTERACTION_VARIANT = 11  # LQ_DEFAULT_INTERACTION_VARIANT
-DEFAULT_SORT_INTERVAL = 4  # LQ_DEFAULT_SORT_INTERVAL
+DEFAULT_SORT_INTERVAL = 8  # LQ_DEFAULT_SORT_INTERVAL
 MAX_SORT_INTERVAL = 16  # LQ_MAX_SORT_INTERVAL
 MAX_PARTICLES = 178956970  # LQ_MAX_PARTICLES
 NOALLOC_MAX_PARTICLES = 2097152  # LQ_NOALLOC_MAX_PARTICLES

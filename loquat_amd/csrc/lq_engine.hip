@@ -1456,9 +1456,9 @@ extern "C" lq_status lq_engine_compact(lq_engine* e, int64_t* removed) {
     if (removed) *removed = 0;
     if (e->runs.empty()) return LQ_OK;
     LQ_TRY(sync_descs(e));
-    // after a sort, a run's particles outside its grid (cell ncell - 1) are
-    // the tail of its segment
-    sort_resident(e);
+    // the table of each run's last sort: its particles outside the grid then
+    // (cell ncell - 1) are the tail of its segment, and the others keep their
+    // keys, order and schedule
     LQ_HIP_TRY(hipStreamSynchronize(e->stream));
     int64_t total = 0;
     for (RunSlot& r : e->runs) {
