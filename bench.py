@@ -38,20 +38,29 @@ BYTES_WRITE_PER_MATERIAL = 52  # a (12) + drho (4) + spin (12) + strain rate (24
 BYTES_PER_CELL = 8             # cell_beg + cell_end
 WORKLOAD_ID = "column_3d dx=5mm npm=1024000"  # the key of this workload in profiles/*.json
 KERNEL_NAMES = {0: "k_interaction_direct", 1: "k_interaction_lds", 2: "k_interaction_unrolled",
-                3: "k_interaction_v3<DescTable, 1, false, false, false, 40, 4, 4>",
-                4: "k_interaction_v3<DescTable, 2, false, false, false, 40, 4, 4>",
-                5: "k_interaction_v3<DescTable, 1, true, false, false, 40, 4, 4>",
-                6: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 4>",
-                7: "k_interaction_v3<DescTable, 2, true, false, false, 32, 4, 4>",
-                8: "k_interaction_v3<DescTable, 2, true, false, false, 48, 4, 4>",
-                9: "k_interaction_v3<DescTable, 2, true, false, false, 40, 8, 4>",
-                10: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 2>",
-                11: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1>"}
+                3: "k_interaction_v3<DescTable, 1, false, false, false, 40, 4, 4, 0, 0>",
+                4: "k_interaction_v3<DescTable, 2, false, false, false, 40, 4, 4, 0, 0>",
+                5: "k_interaction_v3<DescTable, 1, true, false, false, 40, 4, 4, 0, 0>",
+                6: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 4, 0, 0>",
+                7: "k_interaction_v3<DescTable, 2, true, false, false, 32, 4, 4, 0, 0>",
+                8: "k_interaction_v3<DescTable, 2, true, false, false, 48, 4, 4, 0, 0>",
+                9: "k_interaction_v3<DescTable, 2, true, false, false, 40, 8, 4, 0, 0>",
+                10: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 2, 0, 0>",
+                11: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 0, 0>",
+                12: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 12, 1>",
+                13: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 16, 2>",
+                14: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 8, 2>",
+                15: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 0, 2>"}
 # Between a run's sorts both half steps scan its kept cell table with variant
-# 6's shape, in the workgroup size of variants 10 (128 threads) and 11 (64);
-# the predictor half of a sort step scans the fresh table with the variant.
+# 6's shape, in the workgroup size of variants 10 (128 threads) and 11 to 15
+# (64 threads; 12 to 15 with their partial flushes and column orders); the
+# predictor half of a sort step scans the fresh table with the variant.
 def stale_kernel(variant: int) -> str:
-    return f"k_interaction_v3<DescTable, 2, true, false, true, 40, 4, {({10: 2, 11: 1}).get(variant, 4)}>"
+    if variant >= 11:
+        return KERNEL_NAMES[variant].replace("false, false, false", "false, false, true").replace(
+            "true, false, false", "true, false, true")
+    wpb = {10: 2}.get(variant, 4)
+    return f"k_interaction_v3<DescTable, 2, true, false, true, 40, 4, {wpb}, 0, 0>"
 
 
 def roofline_kernel(variant: int) -> str:
@@ -436,6 +445,8 @@ def main() -> None:
                     help="only the batched ensemble (no plain loop): --ensemble-steps timed after --warmup")
     ap.add_argument("--served-only", action="store_true",
                     help="run only the served section (with --serve-requests / --serve-step-scale) and print its JSON")
+    ap.add_argument("--column-order", type=int, default=None, metavar="T",
+                    help="(x, y) column order of the engine's sort: strips of T x-rows (default: the engine's)")
     ap.add_argument("--cell-order", type=int, default=4, metavar="B",
                     help="in-cell order of the step's sort: 0 = by cell only, B > 0 = 2^B z-bins per cell")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
@@ -485,6 +496,9 @@ def main() -> None:
 
     eng = Engine(case.np, p.ncell, 1)
     eng.set_cell_order(a.cell_order)
+    if a.column_order is not None:
+        eng.set_column_order(a.column_order)
+    column_order = eng.column_order()
     sort_interval = eng.sort_interval()
     rid = eng.add(case)
     tuned = None
@@ -525,11 +539,13 @@ def main() -> None:
     # same stream and state, right after the timed loop).
     stage = eng.profile(a.profile_steps)
     info = eng.info(rid)
-    # The same run timed again in the reference's in-cell order (sorted by
-    # cell only, SortParticleByCellIndex): the cost of that exact order.
+    # The same run timed again in the reference's sort order (by cell id
+    # only, SortParticleByCellIndex: no z-bins, x-major columns): the cost of
+    # that exact order.
     ref_order = None
     if not a.no_ref_order:
         eng.set_cell_order(0)
+        eng.set_column_order(0)
         eng.step(a.warmup)
         eng.sync()
         ev2, ev3 = hip.Event(enable_timing=True), hip.Event(enable_timing=True)
@@ -540,7 +556,7 @@ def main() -> None:
         ms_ref = ev2.elapsed_time(ev3) / a.steps
         stage_ref = eng.profile(a.profile_steps)
         ref_order = {"value": round(case.npm / (ms_ref * 1e-3) / 1e6, 3), "ms_per_step": round(ms_ref, 4),
-                     "cell_order_zbits": 0,
+                     "cell_order_zbits": 0, "column_strip_rows": 0,
                      "stage_ms_per_step": {k: round(v, 4) for k, v in
                                            zip(("grid", "adami", "interaction", "integrate", "shepard"), stage_ref)}}
         log(f"reference in-cell order: {ref_order}")
@@ -584,6 +600,7 @@ def main() -> None:
             "npm": case.npm, "np": case.np, "ncell": p.ncell, "dx": 0.005,
             "particle_step": "one material particle advanced one full predictor-corrector step",
             "cell_order_zbits": cell_order,
+            "column_strip_rows": column_order,
             "sort_interval": sort_interval,
             "np_steps_per_s_M": round(case.np * a.steps / (gpu_ms * 1e-3) / 1e6, 3),
             "wall_ms_per_step": round(wall * 1e3 / a.steps, 4),

@@ -390,6 +390,22 @@ lq_status lq_engine_set_graphs(lq_engine* eng, int32_t use_graphs);
  * only the order of the FP32 pair sums changes.  No reference counterpart. */
 lq_status lq_engine_set_cell_order(lq_engine* eng, int32_t zbin_bits);
 
+/* Order of the (x, y) cell columns in the engine's sort (no reference
+ * counterpart; module-level grids keep the reference's cell id order).
+ * strip_rows = 0 or 1: cell id order, x-major; T > 1: strips of T x-rows,
+ * each strip's columns y-major -- (x, y) ranks as (x / T, y, x % T) -- so
+ * that consecutive particles, and the workgroups an XCD runs together,
+ * cover a patch T rows wide instead of a single row, and the neighbour
+ * gathers of a patch share more of the XCD's L2.  The z cells of a column
+ * stay consecutive, cell ids, cell tables and neighbour sets are unchanged,
+ * and so is every result bit (each particle's pairs are summed in the same
+ * order).  Re-sorts the resident runs.  0 .. LQ_MAX_COLUMN_STRIP; default
+ * LQ_DEFAULT_COLUMN_STRIP. */
+#define LQ_DEFAULT_COLUMN_STRIP 8
+#define LQ_MAX_COLUMN_STRIP 64
+lq_status lq_engine_set_column_order(lq_engine* eng, int32_t strip_rows);
+int32_t lq_engine_get_column_order(const lq_engine* eng);
+
 /* Neighbour count of every particle of a run, in the run's input order:
  * material particles get the number of particles (walls and material, not
  * itself) the interaction pairs them with on the resident state, walls 0.
@@ -428,11 +444,11 @@ lq_status lq_engine_profile(lq_engine* eng, int32_t nsteps, double* ms_stage5);
 
 /* Autotuner: times nrep launches of each interaction variant on the engine's
  * resident state and pins the fastest for this engine (all variants produce
- * the same neighbour sets and sum in the same order).  ms_per_variant
+ * the same neighbour sets; their sum orders differ only as stated below).  ms_per_variant
  * (nullable, LQ_INTERACTION_VARIANTS entries) receives the mean launch time
  * of each variant. */
-#define LQ_INTERACTION_VARIANTS 12
-#define LQ_DEFAULT_INTERACTION_VARIANT 11
+#define LQ_INTERACTION_VARIANTS 16
+#define LQ_DEFAULT_INTERACTION_VARIANT 13
 lq_status lq_engine_autotune(lq_engine* eng, int32_t nrep, int32_t* best, double* ms_per_variant);
 
 /* Picks the in-cell order (lq_engine_set_cell_order: 2^b height bins per
@@ -459,17 +475,25 @@ lq_status lq_engine_autotune_cell_order(lq_engine* eng, int32_t nrep, int32_t* b
  *     blocks,
  * 4 = variant 3 evaluating the lists two pairs at a time,
  * 5, 6 = variants 3, 4 where a lane past the end of its list loads nothing
- *        (exec-masked gathers) instead of re-reading its own particle
- *        (the default before variant 11),
- * 7 - 10 = variant 6's code in other shapes, for the autotuner: 7 with
+ *        (exec-masked gathers) instead of re-reading its own particle,
+ * 7 - 11 = variant 6's code in other shapes, for the autotuner: 7 with
  *        32-entry neighbour lists, 8 with 48-entry lists (three workgroups
  *        per CU instead of four), 9 testing eight candidates per load round,
- *        10 in 128-thread workgroups, 11 in 64-thread workgroups (the
- *        default: one wave per workgroup, so a CU holds as many as its
- *        registers allow and a finished wave frees its slot at once).
+ *        10 in 128-thread workgroups, 11 in 64-thread workgroups (one wave
+ *        per workgroup, so a CU holds as many as its registers allow and a
+ *        finished wave frees its slot at once),
+ * 12 - 15 = variant 11 with partial flushes (a full list evaluates only its
+ *        first PF entries and keeps the rest) and the stencil columns in an
+ *        order that keeps the lanes' list lengths alike (opposite columns in
+ *        pairs): 12 PF = 12 with opposite columns pairwise, own column last;
+ *        13 PF = 16 with the edge columns first, then the corners, own
+ *        column last (the default); 14 as 13 with PF = 8; 15 as 13 with full
+ *        flushes.  Variants 3 - 11 sum each particle's pairs in one order,
+ *        13 - 15 in another, 12 in a third (all within FP32 rounding).
  * Variants 3 and up also select the list-based Adami and Shepard scans; the
- * corrector half's stale-table scan and the artificial-pressure kernel use
- * variant 6's shape (in the workgroup size of variants 10 and 11). */
+ * scans of a kept (stale) cell table and the artificial-pressure kernel use
+ * variant 6's shape (in the workgroup size of variants 10 to 15, with their
+ * flushes and column orders). */
 lq_status lq_set_interaction_variant(int32_t variant);
 int32_t lq_get_interaction_variant(void);
 lq_status lq_engine_set_interaction_variant(lq_engine* eng, int32_t variant);

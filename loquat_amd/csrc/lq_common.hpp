@@ -34,6 +34,12 @@ struct RunDesc {
     int32_t step;         // steps completed
     int32_t nsteps;       // steps requested
     int32_t zbits;        // sort key = cell << zbits | z-bin in cell (0: cell only)
+    // Engine sort order of the (x, y) cell columns: 0 = cell id order;
+    // T > 1 = strips of T x-rows, each strip's columns y-major (x, y) ->
+    // (x / T, y, x % T), so that consecutive particles cover a T-row patch
+    // of the grid (cell_rank / cell_of_rank).  Cell ids, the cell table and
+    // every neighbour set are unchanged.
+    int32_t xtile;
     int32_t until_time;   // 1: the run ends when time >= time_max (dynamic dt), else after nsteps
     // The run's cell table is rebuilt every sort-interval steps of its own
     // (engine setting) and after a Shepard filter; in between its particles

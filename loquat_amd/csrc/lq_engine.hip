@@ -436,6 +436,7 @@ struct lq_engine {
     int variant = LQ_DEFAULT_INTERACTION_VARIANT;  // interaction variant (set at creation, by the autotuner or
                                                    // lq_engine_set_interaction_variant)
     int zbits = 4;     // z-bins per cell (log2) in the engine's sort key
+    int xtile = LQ_DEFAULT_COLUMN_STRIP;  // x-rows per strip of the (x, y) column order in the sort key
     int sort_every = LQ_DEFAULT_SORT_INTERVAL;  // a run's cell table is rebuilt every sort_every of its steps
     bool any_dynamic = false;  // some resident run has dynamic-time-step = 1
     int64_t nmat_art = 0;      // material particles of runs with artificial-pressure = 1 (last in mat_idx)
@@ -658,6 +659,7 @@ lq_status upload_descs(lq_engine* e) {
     for (size_t i = 0; i < e->runs.size(); ++i) {
         RunDesc& d = e->runs[i].desc;
         d.zbits = zb;
+        d.xtile = e->xtile;
         e->any_dynamic = e->any_dynamic || d.par.dynamic_time_step == 1;
         h[i] = d;
         f[i] = e->runs[i].failed;
@@ -868,6 +870,21 @@ extern "C" lq_status lq_engine_set_cell_order(lq_engine* e, int32_t zbin_bits) {
     sort_resident(e);
     return last_launch("lq_engine_set_cell_order");
 }
+
+extern "C" lq_status lq_engine_set_column_order(lq_engine* e, int32_t strip_rows) {
+    if (!e || strip_rows < 0 || strip_rows > LQ_MAX_COLUMN_STRIP) {
+        set_error("lq_engine_set_column_order: strip_rows must be 0 .. LQ_MAX_COLUMN_STRIP");
+        return LQ_ERR_INVALID_ARG;
+    }
+    LQ_TRY(sync_descs(e));
+    e->xtile = strip_rows;
+    LQ_TRY(upload_descs(e));
+    drop_graphs(e);
+    // the kept sort keys carry the column ranks: every run is sorted afresh
+    sort_resident(e);
+    return last_launch("lq_engine_set_column_order");
+}
+extern "C" int32_t lq_engine_get_column_order(const lq_engine* e) { return e ? e->xtile : -1; }
 
 static int64_t steps_until(double t, double tmax, double dt) {
     int64_t n = 0;

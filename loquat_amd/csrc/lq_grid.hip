@@ -173,10 +173,22 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* __
     }
 }
 
-// Sort key of particle k: its cell id (layer 0 for walls, 1 for material,
-// index rule of src/adami_boundary.cu:73 and src/loquat.cu:119-124; any axis
-// outside the grid sends the particle to ncell_delete = ncell - 1), shifted
-// left by zbits with the particle's z-bin inside the cell below it.
+// Position of (x, y) column (cx, cy) in the run's sort order (RunDesc::xtile):
+// strips of T x-rows, each strip's columns y-major.  A bijection of
+// [0, nx ny); the last strip holds the nx mod T remaining rows.
+__device__ __forceinline__ int column_rank(const RunDesc& d, int cx, int cy) {
+    const int ny = d.par.cell_num_y, T = d.xtile;
+    if (T <= 1) return cx * ny + cy;
+    const int tx = cx / T, te = min(T, d.par.cell_num_x - tx * T);
+    return tx * T * ny + cy * te + (cx - tx * T);
+}
+
+// Sort key of particle k: its cell's rank (layer 0 for walls, 1 for
+// material, index rule of src/adami_boundary.cu:73 and src/loquat.cu:119-124;
+// any axis outside the grid sends the particle to ncell_delete = ncell - 1),
+// shifted left by zbits with the particle's z-bin inside the cell below it.
+// The rank is the cell id when xtile <= 1 (the reference's order); the z
+// cells of a column stay consecutive either way.
 template <class DP>
 __device__ __forceinline__ uint32_t cell_key(const DP& dp, int k, float4 p) {
     const RunDesc& d = dp.get(dp.run(k));
@@ -186,8 +198,22 @@ __device__ __forceinline__ uint32_t cell_key(const DP& dp, int k, float4 p) {
     const int nx = d.par.cell_num_x, ny = d.par.cell_num_y, nz = d.par.cell_num_z;
     if (cx < 0 || cx >= nx || cy < 0 || cy >= ny || cz < 0 || cz >= nz)
         return (uint32_t)(d.cell_base + d.par.ncell - 1) << d.zbits;
-    return ((uint32_t)(d.cell_base + layer * d.par.ncell_one_layer + (cx * ny + cy) * nz + cz) << d.zbits) |
+    return ((uint32_t)(d.cell_base + layer * d.par.ncell_one_layer + column_rank(d, cx, cy) * nz + cz) << d.zbits) |
            (d.zbits ? zbin_in_cell(p.z, i_h, d.zbits, cz) : 0u);
+}
+
+// Cell id of a global cell rank (inverse of cell_key's rank).
+__device__ __forceinline__ uint32_t cell_of_rank(const RunDesc& d, uint32_t rank) {
+    const int T = d.xtile;
+    const int local = (int)rank - d.cell_base;
+    const int nol = d.par.ncell_one_layer, nz = d.par.cell_num_z, ny = d.par.cell_num_y;
+    if (T <= 1 || local >= 2 * nol) return rank;  // cell id order, or the outside cell
+    const int layer = local / nol, r = local - layer * nol;
+    const int col = r / nz, cz = r - col * nz;
+    const int tx = col / (T * ny), rem = col - tx * T * ny;
+    const int te = min(T, d.par.cell_num_x - tx * T);
+    const int cy = rem / te, cx = tx * T + (rem - cy * te);
+    return (uint32_t)(d.cell_base + layer * nol + (cx * ny + cy) * nz + cz);
 }
 
 // Whether a build re-keys particle k's run (`which`, see GridRekey); the
@@ -253,15 +279,21 @@ __global__ void k_fill2(int32_t* __restrict__ a, int32_t* __restrict__ b, int64_
     }
 }
 
-// Begin/end of every occupied cell; the cell is the key without its low
-// zbits (the z-bin inside the cell).
-__global__ void k_cell_table(const uint32_t* __restrict__ keys, int n, int zbits, int32_t* __restrict__ beg,
+// Begin/end of every occupied cell; the cell's rank is the key without its
+// low zbits (the z-bin inside the cell), its id that of the rank in the
+// run's order.
+template <class DP>
+__global__ void k_cell_table(DP dp, const uint32_t* __restrict__ keys, int n, int zbits, int32_t* __restrict__ beg,
                              int32_t* __restrict__ end) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const uint32_t cell = keys[k] >> zbits;
-    if (k == 0 || (keys[k - 1] >> zbits) != cell) beg[cell] = k;
-    if (k == n - 1 || (keys[k + 1] >> zbits) != cell) end[cell] = k + 1;
+    const uint32_t rank = keys[k] >> zbits;
+    const bool first = k == 0 || (keys[k - 1] >> zbits) != rank;
+    const bool last = k == n - 1 || (keys[k + 1] >> zbits) != rank;
+    if (!first && !last) return;
+    const uint32_t cell = cell_of_rank(dp.get(dp.run(k)), rank);
+    if (first) beg[cell] = k;
+    if (last) end[cell] = k + 1;
 }
 
 // With table positions (engine states): particles of re-keyed runs record
@@ -373,7 +405,7 @@ static void grid_build(const DP& dp, const float4* pospres, int n, int bits, int
     k_hash_count<DP><<<ntiles, RS_THREADS, 0, s>>>(dp, pospres, n, g.keys[0], g.hist, ntiles, g.cell_beg,
                                                    g.cell_end, ncell_total, kept, which);
     radix_passes(g, n, bits, s);
-    k_cell_table<<<blocks_for(n, 256), 256, 0, s>>>(g.keys[g.sorted_slot], n, zbits, g.cell_beg, g.cell_end);
+    k_cell_table<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, g.keys[g.sorted_slot], n, zbits, g.cell_beg, g.cell_end);
 }
 
 void launch_grid_build_one(const RunDesc& d, const float4* pospres, GridBuffers& g, hipStream_t s) {

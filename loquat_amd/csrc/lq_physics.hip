@@ -594,6 +594,17 @@ __device__ __forceinline__ float4 stale_pos(const StaleIn& sn, int k) {
 //
 // B = 2 evaluates the list two entries at a time, eval(j0, act0, j1, act1),
 // so that both gathers are in flight before either pair is computed.
+//
+// PF > 0: when some lane's list is full, only the first PF entries of every
+// list are evaluated and the rest moved to the front (each lane still sums
+// its pairs in candidate order); without it the whole lists are evaluated,
+// every lane padding to the longest.  ORD: the order of the nine (x, y)
+// columns (below); with opposite columns in pairs the lanes' list lengths
+// stay alike, so partial flushes pad little and the lanes' gathers stay in
+// step.  On the 1M-particle column (variant 11's shape): 0.562 ms per
+// launch; PF = 8 alone 0.583 (the lanes' gathers drift apart: more
+// vector-memory traffic); ORD = 1 alone 0.547; ORD = 2 with PF = 8 / 12 / 16
+// / 20: 0.516 / 0.510 / 0.509 / 0.508.
 struct ScanGeom {
     int cix, ciy, ciz, nx, ny, nz, nol, cbase;  // stencil cell and grid
     int czlo, czhi;  // z cells read in every column
@@ -677,7 +688,8 @@ __device__ __forceinline__ bool in_current_stencil(const ScanGeom& g, float4 ps,
            abs(z - cell_coord(ps.z, g.i_h)) <= 1;
 }
 
-template <int B, bool PADDED, int CAP = IC2_CAP, bool STALE = false, int UU = IC2_U, class Eval>
+template <int B, bool PADDED, int CAP = IC2_CAP, bool STALE = false, int UU = IC2_U, int PF = 0, int ORD = 0,
+          class Eval>
 __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer_hi, const ScanGeom& g, float4 ps,
                                             float r2cut, const int32_t* __restrict__ beg,
                                             const int32_t* __restrict__ end, const float4* __restrict__ pp,
@@ -685,29 +697,46 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
     static_assert(B == 1 || B == 2, "batch of 1 or 2");
     static_assert(UU <= SCAN_PAD, "a block of candidates may run SCAN_PAD entries past a range");
     constexpr int U = UU;  // candidates per load round
+    static_assert(PF == 0 || (PF % B == 0 && PF >= UU && PF <= CAP - UU), "partial flush size");
     int count = 0, cnt = 0;
-    auto flush = [&]() {
+    // Evaluates the first min(cnt, m) entries of every lane's list, in list
+    // order, and moves the rest to the front (m = CAP: the whole list).
+    auto flush_m = [&](int m) {
+        const int lim = min(cnt, m);
         // bottom-tested loop: the accumulators flow straight from one
         // iteration to the next (a top-tested loop makes the compiler copy
         // them every iteration)
-        if (__ballot(cnt > 0)) {
+        if (__ballot(lim > 0)) {
             int e = 0;
             do {
                 if constexpr (B == 1) {
-                    const bool act = e < cnt;
+                    const bool act = e < lim;
                     const uint32_t jl = (uint32_t)my[e][lane];
                     eval(act ? jl : dummy, act);
                 } else {
-                    const bool a0 = e < cnt, a1 = e + 1 < cnt;
+                    const bool a0 = e < lim, a1 = e + 1 < lim;
                     const uint32_t j0 = (uint32_t)my[e][lane];
                     const uint32_t j1 = (uint32_t)my[min(e + 1, CAP - 1)][lane];
                     eval(a0 ? j0 : dummy, a0, a1 ? j1 : dummy, a1);
                 }
                 e += B;
-            } while (__ballot(e < cnt));
+            } while (__ballot(e < lim));
         }
-        count += cnt;
-        cnt = 0;
+        count += lim;
+        cnt -= lim;
+        if (m < CAP) {
+            // entries [m, m + cnt) -> [0, cnt); m is the same in every lane
+            for (int e = 0; __ballot(e < cnt); ++e) my[e][lane] = my[min(e + m, CAP - 1)][lane];
+        }
+    };
+    auto flush = [&]() { flush_m(CAP); };
+    // a full list: with PF > 0 only its first PF entries are evaluated (lanes
+    // whose lists are shorter pad less than they would to the longest list)
+    auto flush_full = [&]() {
+        if constexpr (PF > 0)
+            flush_m(PF);
+        else
+            flush();
     };
     const float* __restrict__ pz = reinterpret_cast<const float*>(pp) + 2;  // pz[4 j] = z of particle j
     const int czlo = g.czlo, czhi = g.czhi;
@@ -789,8 +818,21 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
         const int xy = col / nl, layer = layer_lo + col % nl;
         int ox, oy;
         if (xy < 9) {
-            ox = xy / 3 - 1;
-            oy = xy % 3 - 1;
+            // Column order (xy = 3 (ox + 1) + oy + 1, one hex digit per
+            // step).  ORD = 0: x major.  ORD = 1: opposite columns one after
+            // the other, the particle's own column last -- (-1,-1) (1,1)
+            // (-1,0) (1,0) (-1,1) (1,-1) (0,-1) (0,1) (0,0).  ORD = 2: the
+            // four edge columns in opposite pairs, the four corners in
+            // opposite pairs, the own column -- (-1,0) (1,0) (0,-1) (0,1)
+            // (-1,-1) (1,1) (-1,1) (1,-1) (0,0).  A lane near one side of
+            // its cell gains in one column what it loses in the opposite
+            // one, so at any point of the scan the lanes of a wave have
+            // listed similar numbers of hits and flushes pad less.
+            constexpr unsigned long long kOrd = ORD == 1 ? 0x453627180ull
+                                                : ORD == 2 ? 0x462805371ull : 0x876543210ull;
+            const int o = (int)((kOrd >> (4 * xy)) & 0xF);
+            ox = o / 3 - 1;
+            oy = o % 3 - 1;
         } else {
             // the ring of 16 columns two cells out
             const int r = xy - 9;
@@ -901,10 +943,11 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
                 if (q[U - 1].z > zhi) je = min(je, j);
             }
             if (!__ballot(j < je)) break;
-            flush();
+            flush_full();
         }
-        // flush between columns when another column might not fit
-        if (__ballot(cnt > CAP - 3 * U)) flush();
+        // flush between columns when another column might not fit (with
+        // partial flushes a full list is flushed where it fills instead)
+        if (PF == 0 && __ballot(cnt > CAP - 3 * U)) flush();
     }
     flush();
     return count;
@@ -923,7 +966,7 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
 // UU candidates per load round, WPB waves per workgroup (lists are per wave,
 // so a workgroup holds WPB x CAP x 64 entries of LDS).
 template <class DP, int B, bool MASKED = false, bool ART = false, bool STALE = false, int CAP = IC3_CAP,
-          int UU = IC2_U, int WPB = 4>
+          int UU = IC2_U, int WPB = 4, int PF = 0, int ORD = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(STALE ? 4 : 1))) void
 k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ beg, const int32_t* __restrict__ end,
                  IndexMap mats, int n, int32_t* __restrict__ nbr, StaleIn sn = {}) {
@@ -977,11 +1020,11 @@ k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ be
         }
     };
     if constexpr (B == 1) {
-        count = scan_compact<1, DP::padded, CAP, STALE, UU>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+        count = scan_compact<1, DP::padded, CAP, STALE, UU, PF, ORD>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
                                             mat ? (uint32_t)k : 0u,
                                             [&](uint32_t j, bool act) { pair_compute<ART>(pi, load(j, act), c, a); });
     } else {
-        count = scan_compact<2, DP::padded, CAP, STALE, UU>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+        count = scan_compact<2, DP::padded, CAP, STALE, UU, PF, ORD>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
                                             mat ? (uint32_t)k : 0u,
                                             [&](uint32_t j0, bool a0, uint32_t j1, bool a1) {
                                                 const PairIn q0 = load(j0, a0);
@@ -1724,14 +1767,41 @@ void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState&
     launch_adami(DescTable{d, run_of}, st, beg, end, walls, nwalls, variant, stale, nullptr, s);
 }
 
+// Variant 11's shape (64-thread workgroups, 40-entry lists, four candidates
+// per round) with partial flushes of PF entries and column order ORD.
+template <int PF, int ORD, class DP>
+static void launch_v11_shape(const DP& dp, const DevState& st, const DevRates& rt, const int32_t* beg,
+                             const int32_t* end, IndexMap mats, int n, int32_t* nbr, bool art, const StaleIn* stale,
+                             hipStream_t s) {
+    constexpr int P = IC3_CAP, U = IC2_U;
+    const int nb = blocks_for(n, 64);
+    if (stale && art)
+        k_interaction_v3<DP, 2, true, true, true, P, U, 1, PF, ORD><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
+    else if (stale)
+        k_interaction_v3<DP, 2, true, false, true, P, U, 1, PF, ORD><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
+    else if (art)
+        k_interaction_v3<DP, 2, true, true, false, P, U, 1, PF, ORD><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+    else
+        k_interaction_v3<DP, 2, true, false, false, P, U, 1, PF, ORD><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+}
+
 template <class DP>
 static void launch_interaction(const DP& dp, const DevState& st, const DevRates& rt, const int32_t* beg,
                                const int32_t* end, IndexMap mats, int n, int32_t* nbr, int variant, bool art,
                                const StaleIn* stale, hipStream_t s) {
     if (n <= 0) return;
     const int nb = blocks_for(n, 256);
+    // variants 12 to 15: variant 11 with partial flushes of PF entries and
+    // the stencil columns in order ORD (see scan_compact)
+    switch (variant) {
+        case 12: return launch_v11_shape<12, 1>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
+        case 13: return launch_v11_shape<16, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
+        case 14: return launch_v11_shape<8, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
+        case 15: return launch_v11_shape<0, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
+        default: break;
+    }
     // stale tables: variant 6's shape, in the workgroup size of variants 10
-    // and 11
+    // and 11 (12 to 15 above)
     if (stale && art && variant == 11)
         k_interaction_v3<DP, 2, true, true, true, IC3_CAP, IC2_U, 1>
             <<<blocks_for(n, 64), 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);

@@ -276,6 +276,14 @@ class Engine:
         abi.check(abi.lib().lq_engine_neighbour_counts_stale(self.h, rid, out.ctypes.data))
         return out
 
+    def set_column_order(self, strip_rows: int) -> None:
+        """(x, y) column order of the engine's sort: strips of strip_rows x-rows
+        (0 or 1: cell id order); results are unchanged bit for bit."""
+        abi.check(abi.lib().lq_engine_set_column_order(self.h, int(strip_rows)))
+
+    def column_order(self) -> int:
+        return abi.lib().lq_engine_get_column_order(self.h)
+
     def set_sort_interval(self, steps: int) -> None:
         """Sort each run (and rebuild its cell table) every `steps` of its own steps."""
         abi.check(abi.lib().lq_engine_set_sort_interval(self.h, int(steps)))
