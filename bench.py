@@ -399,8 +399,9 @@ def slope_section(steps: int, npm_target: float = 4.0e6, autotune: bool = True, 
     tuned = None
     if autotune:
         bits, _ = eng.autotune_cell_order(3)
+        rows, _ = eng.autotune_column_order(3)
         var, _ = eng.autotune(3)
-        tuned = {"cell_order_zbits": bits, "variant": var}
+        tuned = {"cell_order_zbits": bits, "column_strip_rows": rows, "variant": var}
     eng.step(5)
     eng.sync()
     s = torch.cuda.ExternalStream(eng.stream())
@@ -505,12 +506,16 @@ def main() -> None:
     cell_order = a.cell_order
     if a.autotune:
         cell_order, oms = eng.autotune_cell_order(3)
+        column_order, cms = eng.autotune_column_order(3)
         best, ms = eng.autotune(3)
         tuned = {"variant": best, "kernel": KERNEL_NAMES[best], "launch_ms": [round(m, 4) for m in ms],
                  "cell_order_zbits": cell_order,
-                 "ms_per_cell_order": [round(m, 4) for m in oms]}
+                 "ms_per_cell_order": [round(m, 4) for m in oms],
+                 "column_strip_rows": column_order,
+                 "ms_per_column_order": dict(zip((0, 2, 4, 8, 16), (round(m, 4) for m in cms)))}
         log(f"autotune: cell order {cell_order} (sort + Adami + interaction ms per z-bin bits 0..: "
-            f"{tuned['ms_per_cell_order']}), variant {best}, launch ms per variant {tuned['launch_ms']}")
+            f"{tuned['ms_per_cell_order']}), column strips {column_order} ({tuned['ms_per_column_order']}), "
+            f"variant {best}, launch ms per variant {tuned['launch_ms']}")
     variant = tuned["variant"] if tuned else a.variant
     eng.step(a.warmup)
     eng.sync()

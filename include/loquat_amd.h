@@ -405,6 +405,13 @@ lq_status lq_engine_set_cell_order(lq_engine* eng, int32_t zbin_bits);
 #define LQ_MAX_COLUMN_STRIP 64
 lq_status lq_engine_set_column_order(lq_engine* eng, int32_t strip_rows);
 int32_t lq_engine_get_column_order(const lq_engine* eng);
+/* Picks the column order by timing, on the resident state, the sort, Adami
+ * and the interaction (fresh table) for strips of 0, 2, 4, 8 and 16 rows
+ * (ms_per_candidate, nullable, LQ_COLUMN_ORDER_CANDIDATES entries, in that
+ * order); the order the engine had stays unless another is more than 1 %
+ * faster.  Results do not depend on it.  No reference counterpart. */
+#define LQ_COLUMN_ORDER_CANDIDATES 5
+lq_status lq_engine_autotune_column_order(lq_engine* eng, int32_t nrep, int32_t* best_rows, double* ms_per_candidate);
 
 /* Neighbour count of every particle of a run, in the run's input order:
  * material particles get the number of particles (walls and material, not

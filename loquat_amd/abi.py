@@ -89,6 +89,7 @@ REQ_STATE = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "rejected"}
 INTERACTION_VARIANTS = 16  # LQ_INTERACTION_VARIANTS
 CELL_ORDER_CANDIDATES = 6  # LQ_CELL_ORDER_CANDIDATES
 DEFAULT_INTERACTION_VARIANT = 13  # LQ_DEFAULT_INTERACTION_VARIANT
+COLUMN_ORDER_CANDIDATES = 5  # LQ_COLUMN_ORDER_CANDIDATES
 DEFAULT_SORT_INTERVAL = 8  # LQ_DEFAULT_SORT_INTERVAL
 MAX_SORT_INTERVAL = 16  # LQ_MAX_SORT_INTERVAL
 MAX_PARTICLES = 178956970  # LQ_MAX_PARTICLES
@@ -170,6 +171,7 @@ _PROTOS = {
     "lq_engine_set_sort_interval": (i32, [vp, i32]),
     "lq_engine_set_column_order": (i32, [vp, i32]),
     "lq_engine_get_column_order": (i32, [vp]),
+    "lq_engine_autotune_column_order": (i32, [vp, i32, vp, vp]),
     "lq_engine_get_sort_interval": (i32, [vp]),
     "lq_version": (C.c_char_p, []),
 }

@@ -273,7 +273,8 @@ def served_mix(nreq: int = 48, step_scale: float = 1.0, seed: int = 0, rate: flo
                extremes: bool = False) -> list[tuple[float, Case, int]]:
     """Request stream of the served workload: Poisson arrivals (rate per
     second), column (Drucker-Prager, phi U[20,40] deg, c U[0,5] kPa, aspect
-    U[0.5,4]) or 0.1 m-deep embankment (von Wolffersdorff) 50/50, material
+    U[0.5,4]) or the 30-degree embankment of the slope workload (1.0 m high,
+    3.0 m long, 0.5 m deep; von Wolffersdorff) 50/50, material
     particles log-normal around 30k clipped to [5k, 500k], 2,000-8,000 steps
     scaled by step_scale.  extremes: the first two requests take the clip
     bounds (5k and 500k particles) so that a short stream covers both ends.
@@ -293,7 +294,7 @@ def served_mix(nreq: int = 48, step_scale: float = 1.0, seed: int = 0, rate: flo
             c = column(name=f"req{i}", dx=0.005, col=(w, w, hh), box=(w + 2 * hh, w), quasi2d=False,
                        phi=rng.uniform(20, 40), c=rng.uniform(0, 5e3), shepard=20, nsteps=steps, seed=i)
         else:
-            c = slope_vw(npm_target=npm, nsteps=steps, seed=i, depth=0.1)
+            c = slope_vw(npm_target=npm, nsteps=steps, seed=i)
         out.append((float(arrivals[i]), c, steps))
     return out
 

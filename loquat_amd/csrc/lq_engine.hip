@@ -1297,6 +1297,77 @@ extern "C" lq_status lq_engine_autotune_cell_order(lq_engine* e, int32_t nrep, i
     return LQ_OK;
 }
 
+extern "C" lq_status lq_engine_autotune_column_order(lq_engine* e, int32_t nrep, int32_t* best_rows,
+                                                     double* ms_per_candidate) {
+    static constexpr int kRows[LQ_COLUMN_ORDER_CANDIDATES] = {0, 2, 4, 8, 16};
+    if (!e || nrep <= 0) {
+        set_error("lq_engine_autotune_column_order: invalid argument");
+        return LQ_ERR_INVALID_ARG;
+    }
+    if (e->runs.empty() || e->nmat_total == 0) {
+        set_error("lq_engine_autotune_column_order: no material particles resident");
+        return LQ_ERR_STATE;
+    }
+    Buffers& b = e->b;
+    hipStream_t s = e->stream;
+    LQ_HIP_TRY(hipStreamSynchronize(s));
+    const IndexMap walls{b.wall_idx, 0}, mats{b.mat_idx, 0};
+    const int var = e->variant;
+    // what the column order changes: where the sorted particles lie, hence
+    // the L2 reuse of Adami's and the interaction's gathers (both scans on a
+    // fresh table), and the sort itself
+    auto once = [&]() {
+        sort_resident(e);
+        launch_adami_table(b.desc, b.run_of, b.S[e->cur], b.grid.cell_beg, b.grid.cell_end, walls,
+                           (int)e->nwall_total, var, nullptr, s);
+        launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
+                                 (int)e->nmat_total, (int)e->nmat_art, nullptr, var, nullptr, s);
+    };
+    hipEvent_t ev[2];
+    LQ_HIP_TRY(hipEventCreate(&ev[0]));
+    LQ_HIP_TRY(hipEventCreate(&ev[1]));
+    const int start = e->xtile;
+    double best_ms = 1e30, start_ms = -1.0;
+    int best_r = start;
+    lq_status st = LQ_OK;
+    for (int c = 0; c < LQ_COLUMN_ORDER_CANDIDATES && st == LQ_OK; ++c) {
+        st = lq_engine_set_column_order(e, kRows[c]);
+        if (st != LQ_OK) break;
+        once();  // warm-up, and the resident state in this order
+        hipError_t err = hipEventRecord(ev[0], s);
+        for (int r = 0; r < nrep && err == hipSuccess; ++r) once();
+        if (err == hipSuccess) err = hipEventRecord(ev[1], s);
+        if (err == hipSuccess) err = hipEventSynchronize(ev[1]);
+        float ms = 0.f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        if (err != hipSuccess) {
+            set_error(std::string("lq_engine_autotune_column_order: ") + hipGetErrorString(err));
+            st = LQ_ERR_HIP;
+            break;
+        }
+        const double per = ms / nrep;
+        if (ms_per_candidate) ms_per_candidate[c] = per;
+        if (kRows[c] == start) start_ms = per;
+        if (per < best_ms) {
+            best_ms = per;
+            best_r = kRows[c];
+        }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (st == LQ_OK) st = last_launch("lq_engine_autotune_column_order");
+    if (st != LQ_OK) {
+        (void)lq_engine_set_column_order(e, start);  // leave the engine as it was
+        return st;
+    }
+    // the order the engine had stays unless another is clearly faster (1 %)
+    if (start_ms > 0.0 && start_ms <= 1.01 * best_ms) best_r = start;
+    LQ_TRY(lq_engine_set_column_order(e, best_r));
+    LQ_HIP_TRY(hipStreamSynchronize(s));
+    if (best_rows) *best_rows = best_r;
+    return LQ_OK;
+}
+
 extern "C" lq_status lq_engine_profile(lq_engine* e, int32_t nsteps, double* ms_stage5) {
     if (!e || nsteps <= 0 || !ms_stage5) {
         set_error("lq_engine_profile: invalid argument");

@@ -326,6 +326,15 @@ class Engine:
         abi.check(abi.lib().lq_engine_autotune_cell_order(self.h, int(nrep), C.byref(best), ms))
         return best.value, list(ms)
 
+    def autotune_column_order(self, nrep: int = 3) -> tuple[int, list[float]]:
+        """Pins the fastest (x, y) column order (strips of 0, 2, 4, 8, 16
+        rows) for this engine, timing sort, Adami and interaction per
+        candidate; results do not depend on it."""
+        best = C.c_int32()
+        ms = (C.c_double * abi.COLUMN_ORDER_CANDIDATES)()
+        abi.check(abi.lib().lq_engine_autotune_column_order(self.h, int(nrep), C.byref(best), ms))
+        return best.value, list(ms)
+
     def profile(self, nsteps: int) -> list[float]:
         ms = (C.c_double * 5)()
         abi.check(abi.lib().lq_engine_profile(self.h, int(nsteps), ms))

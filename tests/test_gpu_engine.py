@@ -608,3 +608,20 @@ def test_autotune_cell_order_keeps_neighbour_sets(gpu):
     assert e.info(rid)["steps"] == 6 and e.info(rid)["finished"]
     assert np.isfinite(e.download(rid)["velrhop"]).all()
     e.close()
+
+
+@pytest.mark.gpu
+def test_autotune_column_order_keeps_results(gpu):
+    """The column-order tuner times its five candidates, pins one of them and
+    leaves results bit-identical to an untuned engine's."""
+    c = cases.perturb(cases.small_column_3d(seed=61), seed=62)
+    e, (rid,) = _engine([c], 6)
+    rows, ms = e.autotune_column_order(2)
+    assert rows in (0, 2, 4, 8, 16) and e.column_order() == rows and len(ms) == 5 and all(m > 0 for m in ms)
+    e.step(6)
+    tuned = e.download(rid)
+    e.close()
+    e, (rid,) = _engine([c], 6)
+    e.step(6)
+    assert _same(tuned, e.download(rid))
+    e.close()
