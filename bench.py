@@ -446,6 +446,8 @@ def main() -> None:
                     help="only the batched ensemble (no plain loop): --ensemble-steps timed after --warmup")
     ap.add_argument("--served-only", action="store_true",
                     help="run only the served section (with --serve-requests / --serve-step-scale) and print its JSON")
+    ap.add_argument("--sort-interval", type=int, default=None, metavar="K",
+                    help="re-sort each run every K of its steps (default: the engine's)")
     ap.add_argument("--column-order", type=int, default=None, metavar="T",
                     help="(x, y) column order of the engine's sort: strips of T x-rows (default: the engine's)")
     ap.add_argument("--cell-order", type=int, default=4, metavar="B",
@@ -499,6 +501,8 @@ def main() -> None:
     eng.set_cell_order(a.cell_order)
     if a.column_order is not None:
         eng.set_column_order(a.column_order)
+    if a.sort_interval is not None:
+        eng.set_sort_interval(a.sort_interval)
     column_order = eng.column_order()
     sort_interval = eng.sort_interval()
     rid = eng.add(case)
