@@ -450,11 +450,12 @@ int32_t lq_engine_get_sort_interval(const lq_engine* eng);
 lq_status lq_engine_profile(lq_engine* eng, int32_t nsteps, double* ms_stage5);
 
 /* Autotuner: times nrep launches of each interaction variant on the engine's
- * resident state and pins the fastest for this engine (all variants produce
+ * resident state and pins the fastest for this engine (the engine's variant
+ * stays unless another is more than 1 % faster) (all variants produce
  * the same neighbour sets; their sum orders differ only as stated below).  ms_per_variant
  * (nullable, LQ_INTERACTION_VARIANTS entries) receives the mean launch time
  * of each variant. */
-#define LQ_INTERACTION_VARIANTS 16
+#define LQ_INTERACTION_VARIANTS 17
 #define LQ_DEFAULT_INTERACTION_VARIANT 13
 lq_status lq_engine_autotune(lq_engine* eng, int32_t nrep, int32_t* best, double* ms_per_variant);
 
@@ -495,12 +496,16 @@ lq_status lq_engine_autotune_cell_order(lq_engine* eng, int32_t nrep, int32_t* b
  *        pairs): 12 PF = 12 with opposite columns pairwise, own column last;
  *        13 PF = 16 with the edge columns first, then the corners, own
  *        column last (the default); 14 as 13 with PF = 8; 15 as 13 with full
- *        flushes.  Variants 3 - 11 sum each particle's pairs in one order,
- *        13 - 15 in another, 12 in a third (all within FP32 rounding).
+ *        flushes,
+ * 16 = 13 with two lanes per particle (32 particles per wave, the lane pair
+ *        taking a column's candidates alternately, the two partial sums
+ *        added at the end).  Variants 3 - 11 sum each particle's pairs in one
+ *        order, 13 - 15 in another, 12 and 16 in their own (all within FP32
+ *        rounding).
  * Variants 3 and up also select the list-based Adami and Shepard scans; the
  * scans of a kept (stale) cell table and the artificial-pressure kernel use
- * variant 6's shape (in the workgroup size of variants 10 to 15, with their
- * flushes and column orders). */
+ * variant 6's shape (in the workgroup size of variants 10 to 16, with their
+ * flushes, column orders and lanes per particle). */
 lq_status lq_set_interaction_variant(int32_t variant);
 int32_t lq_get_interaction_variant(void);
 lq_status lq_engine_set_interaction_variant(lq_engine* eng, int32_t variant);

@@ -38,6 +38,10 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 __device__ __forceinline__ int xcd_thread() { return xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; }
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// The value of lane ^ 1 (DPP quad permutation 1, 0, 3, 2).
+__device__ __forceinline__ float swap_pair(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 __device__ __forceinline__ tensor2d tadd(const tensor2d& a, const tensor2d& b) {
@@ -689,14 +693,20 @@ __device__ __forceinline__ bool in_current_stencil(const ScanGeom& g, float4 ps,
 }
 
 template <int B, bool PADDED, int CAP = IC2_CAP, bool STALE = false, int UU = IC2_U, int PF = 0, int ORD = 0,
-          class Eval>
+          int LPP = 1, class Eval>
 __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer_hi, const ScanGeom& g, float4 ps,
                                             float r2cut, const int32_t* __restrict__ beg,
                                             const int32_t* __restrict__ end, const float4* __restrict__ pp,
                                             int32_t (*my)[64], int lane, uint32_t dummy, Eval&& eval) {
     static_assert(B == 1 || B == 2, "batch of 1 or 2");
-    static_assert(UU <= SCAN_PAD, "a block of candidates may run SCAN_PAD entries past a range");
-    constexpr int U = UU;  // candidates per load round
+    static_assert(LPP == 1 || LPP == 2, "one or two lanes per particle");
+    static_assert(UU * LPP <= SCAN_PAD, "a block of candidates may run SCAN_PAD entries past a range");
+    constexpr int U = UU;  // candidates per load round (per lane)
+    // LPP = 2: the two lanes of a particle (lane ^ 1) take a column's
+    // candidates alternately (sub = 0: even offsets from the block start,
+    // 1: odd), so a wave holds 32 particles
+    constexpr int S = U * LPP;  // candidates per round of a particle's lanes
+    const int sub = LPP == 2 ? (lane & 1) : 0;
     static_assert(PF == 0 || (PF % B == 0 && PF >= UU && PF <= CAP - UU), "partial flush size");
     int count = 0, cnt = 0;
     // Evaluates the first min(cnt, m) entries of every lane's list, in list
@@ -800,12 +810,12 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
             int adv = 0;
 #pragma unroll
             for (int m = 1; m <= 4; ++m) {
-                const int last = j + m * U - 1;
+                const int last = j + m * S - 1;
                 const float zm = ldo(pz, 4u * (uint32_t)min(last, e0 - 1));
-                if (last < e0 && zm < zlo) adv = m * U;
+                if (last < e0 && zm < zlo) adv = m * S;
             }
             j += adv;
-            if (adv < 4 * U) break;
+            if (adv < 4 * S) break;
         }
         return j;
     };
@@ -902,13 +912,13 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
                 float4 q[U];
                 if (PADDED) {
                     // one offset for the four loads; a finished lane re-reads
-                    // [je, je + 3], inside the padded arrays
-                    const uint32_t jj = (uint32_t)min(j, je);
+                    // [je, je + S), inside the padded arrays
+                    const uint32_t jj = (uint32_t)(min(j, je) + sub);
 #pragma unroll
-                    for (int u = 0; u < U; ++u) q[u] = ldo(pp + u, jj);
+                    for (int u = 0; u < U; ++u) q[u] = ldo(pp + LPP * u, jj);
                 } else {
 #pragma unroll
-                    for (int u = 0; u < U; ++u) q[u] = ldo(pp, min((uint32_t)(j + u), jlast));
+                    for (int u = 0; u < U; ++u) q[u] = ldo(pp, min((uint32_t)(j + sub + LPP * u), jlast));
                 }
                 int hit[U];
                 unsigned long long check = 0ull;
@@ -917,7 +927,7 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
                     // both tests evaluated, combined without a branch
                     const float r2 = r2_fp32(ps.x - q[u].x, ps.y - q[u].y, ps.z - q[u].z);
                     const int close = r2 < r2cut ? 1 : 0;
-                    const int inside = (j + u < je) ? 1 : 0;
+                    const int inside = (j + sub + LPP * u < je) ? 1 : 0;
                     hit[u] = close & inside;
                     // Closer than (1 - eps) cell (so along every axis), two
                     // particles' cells differ by at most one (the FP32
@@ -935,10 +945,10 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    my[cnt][lane] = j + u;
+                    my[cnt][lane] = j + sub + LPP * u;
                     cnt += hit[u];
                 }
-                j += U;
+                j += S;
                 // every later particle of the column lies above the window
                 if (q[U - 1].z > zhi) je = min(je, j);
             }
@@ -966,12 +976,13 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
 // UU candidates per load round, WPB waves per workgroup (lists are per wave,
 // so a workgroup holds WPB x CAP x 64 entries of LDS).
 template <class DP, int B, bool MASKED = false, bool ART = false, bool STALE = false, int CAP = IC3_CAP,
-          int UU = IC2_U, int WPB = 4, int PF = 0, int ORD = 0>
+          int UU = IC2_U, int WPB = 4, int PF = 0, int ORD = 0, int LPP = 1>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(STALE ? 4 : 1))) void
 k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ beg, const int32_t* __restrict__ end,
                  IndexMap mats, int n, int32_t* __restrict__ nbr, StaleIn sn = {}) {
     __shared__ int32_t lists[WPB][CAP][64];
-    const int t = xcd_thread();
+    // LPP lanes per particle: adjacent lanes of one wave
+    const int t = xcd_thread() / LPP;
     const int k = t < n ? mats(t) : -1;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     bool mat = false;
@@ -1020,11 +1031,11 @@ k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ be
         }
     };
     if constexpr (B == 1) {
-        count = scan_compact<1, DP::padded, CAP, STALE, UU, PF, ORD>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+        count = scan_compact<1, DP::padded, CAP, STALE, UU, PF, ORD, LPP>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
                                             mat ? (uint32_t)k : 0u,
                                             [&](uint32_t j, bool act) { pair_compute<ART>(pi, load(j, act), c, a); });
     } else {
-        count = scan_compact<2, DP::padded, CAP, STALE, UU, PF, ORD>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+        count = scan_compact<2, DP::padded, CAP, STALE, UU, PF, ORD, LPP>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
                                             mat ? (uint32_t)k : 0u,
                                             [&](uint32_t j0, bool a0, uint32_t j1, bool a1) {
                                                 const PairIn q0 = load(j0, a0);
@@ -1033,7 +1044,18 @@ k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ be
                                                 pair_compute<ART>(pi, q1, c, a);
                                             });
     }
-    if (mat) {
+    if constexpr (LPP == 2) {
+        // the particle's two partial sums: each lane adds its neighbour's
+        // (lane ^ 1), so both hold the same total
+        a.ax += swap_pair(a.ax);
+        a.ay += swap_pair(a.ay);
+        a.az += swap_pair(a.az);
+        a.dvg += swap_pair(a.dvg);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) a.L[q] += swap_pair(a.L[q]);
+        count += __float_as_int(swap_pair(__int_as_float(count)));
+    }
+    if (mat && (LPP == 1 || (lane & 1) == 0)) {
         store_rates(*dptr, rt, k, a);
         if (nbr) nbr[k] = count - (self_in ? 1 : 0);
     }
@@ -1769,20 +1791,20 @@ void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState&
 
 // Variant 11's shape (64-thread workgroups, 40-entry lists, four candidates
 // per round) with partial flushes of PF entries and column order ORD.
-template <int PF, int ORD, class DP>
+template <int PF, int ORD, int LPP = 1, class DP>
 static void launch_v11_shape(const DP& dp, const DevState& st, const DevRates& rt, const int32_t* beg,
                              const int32_t* end, IndexMap mats, int n, int32_t* nbr, bool art, const StaleIn* stale,
                              hipStream_t s) {
     constexpr int P = IC3_CAP, U = IC2_U;
-    const int nb = blocks_for(n, 64);
+    const int nb = blocks_for((int64_t)LPP * n, 64);
     if (stale && art)
-        k_interaction_v3<DP, 2, true, true, true, P, U, 1, PF, ORD><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
+        k_interaction_v3<DP, 2, true, true, true, P, U, 1, PF, ORD, LPP><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
     else if (stale)
-        k_interaction_v3<DP, 2, true, false, true, P, U, 1, PF, ORD><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
+        k_interaction_v3<DP, 2, true, false, true, P, U, 1, PF, ORD, LPP><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
     else if (art)
-        k_interaction_v3<DP, 2, true, true, false, P, U, 1, PF, ORD><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+        k_interaction_v3<DP, 2, true, true, false, P, U, 1, PF, ORD, LPP><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else
-        k_interaction_v3<DP, 2, true, false, false, P, U, 1, PF, ORD><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+        k_interaction_v3<DP, 2, true, false, false, P, U, 1, PF, ORD, LPP><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
 }
 
 template <class DP>
@@ -1791,13 +1813,15 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
                                const StaleIn* stale, hipStream_t s) {
     if (n <= 0) return;
     const int nb = blocks_for(n, 256);
-    // variants 12 to 15: variant 11 with partial flushes of PF entries and
-    // the stencil columns in order ORD (see scan_compact)
+    // variants 12 to 16: variant 11 with partial flushes of PF entries and
+    // the stencil columns in order ORD (see scan_compact); 16 as 13 with two
+    // lanes per particle (32 particles per wave)
     switch (variant) {
         case 12: return launch_v11_shape<12, 1>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         case 13: return launch_v11_shape<16, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         case 14: return launch_v11_shape<8, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         case 15: return launch_v11_shape<0, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
+        case 16: return launch_v11_shape<16, 2, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         default: break;
     }
     // stale tables: variant 6's shape, in the workgroup size of variants 10
