@@ -450,9 +450,9 @@ int32_t lq_engine_get_sort_interval(const lq_engine* eng);
 lq_status lq_engine_profile(lq_engine* eng, int32_t nsteps, double* ms_stage5);
 
 /* Autotuner: times nrep launches of each interaction variant on the engine's
- * resident state and pins the fastest for this engine (the engine's variant
- * stays unless another is more than 1 % faster) (all variants produce
- * the same neighbour sets; their sum orders differ only as stated below).  ms_per_variant
+ * resident state and pins the fastest for this engine; the engine's variant
+ * stays unless another is more than 1 % faster.  All variants produce the
+ * same neighbour sets; their sum orders differ only as stated below.  ms_per_variant
  * (nullable, LQ_INTERACTION_VARIANTS entries) receives the mean launch time
  * of each variant. */
 #define LQ_INTERACTION_VARIANTS 17

@@ -180,7 +180,9 @@ def test_autotune_picks_a_variant_and_keeps_results(gpu):
     c = cases.perturb(cases.small_column_3d(seed=51), seed=52)
     e, (rid,) = _engine([c], 6)
     best, ms = e.autotune(2)
-    assert best in range(abi.INTERACTION_VARIANTS) and all(m > 0 for m in ms) and ms[best] == min(ms)
+    # the fastest, or the engine's default variant when that is within 1 %
+    assert best in range(abi.INTERACTION_VARIANTS) and all(m > 0 for m in ms)
+    assert ms[best] == min(ms) or (best == abi.DEFAULT_INTERACTION_VARIANT and ms[best] <= 1.01 * min(ms))
     e.step(6)
     tuned = e.download(rid)
     e.close()
