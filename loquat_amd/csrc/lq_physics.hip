@@ -1063,11 +1063,13 @@ k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ be
 
 // Adami wall stress with the list-based scan (material layer only; a wall is
 // never in its own candidate stream).  Same sums as k_adami, same order.
-template <class DP, bool STALE = false>
+// WPB waves per workgroup; PF and ORD as in scan_compact (partial flushes,
+// column order).
+template <class DP, bool STALE = false, int WPB = 4, int PF = 0, int ORD = 0>
 __global__ __launch_bounds__(256) void k_adami_lds(DP dp, DevState st, const int32_t* __restrict__ beg,
                                                    const int32_t* __restrict__ end, IndexMap walls, int n,
                                                    StaleIn sn = {}, int32_t* __restrict__ nbr = nullptr) {
-    __shared__ int32_t lists[4][IC2_CAP][64];
+    __shared__ int32_t lists[WPB][IC2_CAP][64];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;  // round-robin XCDs, as k_adami
     const int k = t < n ? walls(t) : -1;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1098,7 +1100,7 @@ __global__ __launch_bounds__(256) void k_adami_lds(DP dp, DevState st, const int
     }
     float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
     float px = 0.f, py = 0.f, pz = 0.f, wsum = 0.f;
-    const int cnt = scan_compact<1, DP::padded, IC2_CAP, STALE>(on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w],
+    const int cnt = scan_compact<1, DP::padded, IC2_CAP, STALE, IC2_U, PF, ORD>(on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w],
                                                                  lane, on ? (uint32_t)k : 0u,
                              [&](uint32_t j, bool act) {
         const float dx = (float)(xi - ldo(st.xd, j)), dy = (float)(yi - ldo(st.yd, j)),
@@ -1772,6 +1774,15 @@ template <class DP>
 static void launch_adami(const DP& dp, const DevState& st, const int32_t* beg, const int32_t* end, IndexMap walls,
                          int n, int variant, const StaleIn* stale, int32_t* nbr, hipStream_t s) {
     if (n <= 0) return;
+    if (variant >= 12) {
+        // with the variants' partial flushes and edge-first column order, one
+        // wave per workgroup
+        if (stale)
+            k_adami_lds<DP, true, 1, 16, 2><<<blocks_for(n, 64), 64, 0, s>>>(dp, st, beg, end, walls, n, *stale, nbr);
+        else
+            k_adami_lds<DP, false, 1, 16, 2><<<blocks_for(n, 64), 64, 0, s>>>(dp, st, beg, end, walls, n, StaleIn{}, nbr);
+        return;
+    }
     if (stale)
         k_adami_lds<DP, true><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, beg, end, walls, n, *stale, nbr);
     else if (variant >= 3)
