@@ -38,31 +38,32 @@ BYTES_WRITE_PER_MATERIAL = 52  # a (12) + drho (4) + spin (12) + strain rate (24
 BYTES_PER_CELL = 8             # cell_beg + cell_end
 WORKLOAD_ID = "column_3d dx=5mm npm=1024000"  # the key of this workload in profiles/*.json
 KERNEL_NAMES = {0: "k_interaction_direct", 1: "k_interaction_lds", 2: "k_interaction_unrolled",
-                3: "k_interaction_v3<DescTable, 1, false, false, false, 40, 4, 4, 0, 0, 1>",
-                4: "k_interaction_v3<DescTable, 2, false, false, false, 40, 4, 4, 0, 0, 1>",
-                5: "k_interaction_v3<DescTable, 1, true, false, false, 40, 4, 4, 0, 0, 1>",
-                6: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 4, 0, 0, 1>",
-                7: "k_interaction_v3<DescTable, 2, true, false, false, 32, 4, 4, 0, 0, 1>",
-                8: "k_interaction_v3<DescTable, 2, true, false, false, 48, 4, 4, 0, 0, 1>",
-                9: "k_interaction_v3<DescTable, 2, true, false, false, 40, 8, 4, 0, 0, 1>",
-                10: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 2, 0, 0, 1>",
-                11: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 0, 0, 1>",
-                12: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 12, 1, 1>",
-                13: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 16, 2, 1>",
-                14: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 8, 2, 1>",
-                15: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 0, 2, 1>",
-                16: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 16, 2, 2>"}
+                3: "k_interaction_v3<DescTable, 1, false, false, false, 40, 4, 4, 0, 0, 1, false>",
+                4: "k_interaction_v3<DescTable, 2, false, false, false, 40, 4, 4, 0, 0, 1, false>",
+                5: "k_interaction_v3<DescTable, 1, true, false, false, 40, 4, 4, 0, 0, 1, false>",
+                6: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 4, 0, 0, 1, false>",
+                7: "k_interaction_v3<DescTable, 2, true, false, false, 32, 4, 4, 0, 0, 1, false>",
+                8: "k_interaction_v3<DescTable, 2, true, false, false, 48, 4, 4, 0, 0, 1, false>",
+                9: "k_interaction_v3<DescTable, 2, true, false, false, 40, 8, 4, 0, 0, 1, false>",
+                10: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 2, 0, 0, 1, false>",
+                11: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 0, 0, 1, false>",
+                12: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 12, 1, 1, false>",
+                13: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 16, 2, 1, false>",
+                14: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 8, 2, 1, false>",
+                15: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 0, 2, 1, false>",
+                16: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 16, 2, 2, false>",
+                17: "k_interaction_v3<DescTable, 2, true, false, false, 32, 4, 1, 16, 2, 1, true>"}
 # Between a run's sorts both half steps scan its kept cell table with variant
-# 6's shape, in the workgroup size of variants 10 (128 threads) and 11 to 16
-# (64 threads; 12 to 16 with their partial flushes and column orders, 16 with
-# two lanes per particle); the predictor half of a sort step scans the fresh
-# table with the variant.
+# 6's shape, in the workgroup size of variants 10 (128 threads) and 11 to 17
+# (64 threads; 12 to 17 with their partial flushes and column orders, 16 with
+# two lanes per particle, 17 with staged candidates); the predictor half of a
+# sort step scans the fresh table with the variant.
 def stale_kernel(variant: int) -> str:
     if variant >= 11:
         return KERNEL_NAMES[variant].replace("false, false, false", "false, false, true").replace(
             "true, false, false", "true, false, true")
     wpb = {10: 2}.get(variant, 4)
-    return f"k_interaction_v3<DescTable, 2, true, false, true, 40, 4, {wpb}, 0, 0, 1>"
+    return f"k_interaction_v3<DescTable, 2, true, false, true, 40, 4, {wpb}, 0, 0, 1, false>"
 
 
 def roofline_kernel(variant: int) -> str:

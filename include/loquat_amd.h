@@ -455,7 +455,7 @@ lq_status lq_engine_profile(lq_engine* eng, int32_t nsteps, double* ms_stage5);
  * same neighbour sets; their sum orders differ only as stated below.  ms_per_variant
  * (nullable, LQ_INTERACTION_VARIANTS entries) receives the mean launch time
  * of each variant. */
-#define LQ_INTERACTION_VARIANTS 17
+#define LQ_INTERACTION_VARIANTS 18
 #define LQ_DEFAULT_INTERACTION_VARIANT 13
 lq_status lq_engine_autotune(lq_engine* eng, int32_t nrep, int32_t* best, double* ms_per_variant);
 
@@ -499,13 +499,17 @@ lq_status lq_engine_autotune_cell_order(lq_engine* eng, int32_t nrep, int32_t* b
  *        flushes,
  * 16 = 13 with two lanes per particle (32 particles per wave, the lane pair
  *        taking a column's candidates alternately, the two partial sums
- *        added at the end).  Variants 3 - 11 sum each particle's pairs in one
- *        order, 13 - 15 in another, 12 and 16 in their own (all within FP32
- *        rounding).
+ *        added at the end),
+ * 17 = 13 with 32-entry lists and the candidates of each stencil column
+ *        staged in LDS by the wave (chunks of 128 consecutive particles from
+ *        its lanes' lowest pending candidate, loaded coalesced; chunk bounds
+ *        by wavefront-shuffle minimum / maximum) and tested from there.
+ *        Variants 3 - 11 sum each particle's pairs in one order, 13 - 15 and
+ *        17 in another, 12 and 16 in their own (all within FP32 rounding).
  * Variants 3 and up also select the list-based Adami and Shepard scans; the
  * scans of a kept (stale) cell table and the artificial-pressure kernel use
- * variant 6's shape (in the workgroup size of variants 10 to 16, with their
- * flushes, column orders and lanes per particle). */
+ * variant 6's shape (in the workgroup size of variants 10 to 17, with their
+ * flushes, column orders, lanes per particle and staging). */
 lq_status lq_set_interaction_variant(int32_t variant);
 int32_t lq_get_interaction_variant(void);
 lq_status lq_engine_set_interaction_variant(lq_engine* eng, int32_t variant);

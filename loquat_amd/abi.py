@@ -86,7 +86,7 @@ class LqServerConfig(C.Structure):
 
 
 REQ_STATE = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "rejected"}
-INTERACTION_VARIANTS = 17  # LQ_INTERACTION_VARIANTS
+INTERACTION_VARIANTS = 18  # LQ_INTERACTION_VARIANTS
 CELL_ORDER_CANDIDATES = 6  # LQ_CELL_ORDER_CANDIDATES
 DEFAULT_INTERACTION_VARIANT = 13  # LQ_DEFAULT_INTERACTION_VARIANT
 COLUMN_ORDER_CANDIDATES = 5  # LQ_COLUMN_ORDER_CANDIDATES

@@ -38,6 +38,24 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 __device__ __forceinline__ int xcd_thread() { return xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; }
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// Minimum / maximum of v over the 64 lanes (every lane must call).
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+// Slots per wave of the staged scan (scan_compact, STG): a chunk stages
+// STG_SLOTS consecutive candidates.  128 x 16 B with 32-entry lists keeps a
+// one-wave workgroup at 10 KB of LDS: sixteen per CU, as many as registers
+// allow (with 4 more slots, or 40-entry lists, fifteen or thirteen fit, and
+// the launch slows by 9 % and 15 %).
+constexpr int STG_SLOTS = 128;
+
 // The value of lane ^ 1 (DPP quad permutation 1, 0, 3, 2).
 __device__ __forceinline__ float swap_pair(float x) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
@@ -693,11 +711,12 @@ __device__ __forceinline__ bool in_current_stencil(const ScanGeom& g, float4 ps,
 }
 
 template <int B, bool PADDED, int CAP = IC2_CAP, bool STALE = false, int UU = IC2_U, int PF = 0, int ORD = 0,
-          int LPP = 1, class Eval>
+          int LPP = 1, bool STG = false, class Eval>
 __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer_hi, const ScanGeom& g, float4 ps,
                                             float r2cut, const int32_t* __restrict__ beg,
                                             const int32_t* __restrict__ end, const float4* __restrict__ pp,
-                                            int32_t (*my)[64], int lane, uint32_t dummy, Eval&& eval) {
+                                            int32_t (*my)[64], int lane, uint32_t dummy, Eval&& eval,
+                                            float4* stage = nullptr) {
     static_assert(B == 1 || B == 2, "batch of 1 or 2");
     static_assert(LPP == 1 || LPP == 2, "one or two lanes per particle");
     static_assert(UU * LPP <= SCAN_PAD, "a block of candidates may run SCAN_PAD entries past a range");
@@ -904,6 +923,69 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
             }
         }
         const uint32_t jlast = (uint32_t)max(je - 1, 0);
+        if constexpr (STG && PADDED && LPP == 1) {
+            // Staged scan: the wave copies the candidates of its lanes'
+            // windows in chunks of STG_SLOTS consecutive particles (the
+            // chunk starts at the lowest pending candidate of any lane) into
+            // LDS with coalesced loads, and each lane tests its own blocks of
+            // the chunk from there; a block whose candidates reach past the
+            // chunk waits for the next one.  Same candidates, same order, same
+            // lists.
+            while (true) {
+                if (__ballot(cnt > CAP - U)) {
+                    flush_full();
+                    continue;
+                }
+                const bool pend = j < je;
+                if (!__ballot(pend)) break;
+                const int base = wave_min_i32(pend ? j : INT_MAX);
+                const int top = wave_max_i32(pend ? je : 0);  // every loaded index stays below it
+#pragma unroll
+                for (int h = 0; h < STG_SLOTS / 64; ++h)
+                    stage[h * 64 + lane] = pp[min(base + h * 64 + lane, top - 1)];
+                // the wave's own LDS writes precede its reads (one wave per slot)
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const int lim = base + STG_SLOTS;
+                while (true) {
+                    if (__ballot(cnt > CAP - U)) flush_full();
+                    const bool can = j < je && min(j + U, je) <= lim;
+                    if (!__ballot(can)) break;
+                    if (can) {
+                        // slots past the chunk hold candidates past je (not
+                        // hits): the index is clamped into the chunk
+                        float4 q[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) q[u] = stage[min(j - base + u, STG_SLOTS - 1)];
+                        int hit[U];
+                        unsigned long long check = 0ull;
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const float r2 = r2_fp32(ps.x - q[u].x, ps.y - q[u].y, ps.z - q[u].z);
+                            hit[u] = (r2 < r2cut ? 1 : 0) & ((j + u < je) ? 1 : 0);
+                            if constexpr (STALE) check |= __ballot(r2 >= g.ext2 && r2 < r2cut);
+                        }
+                        if constexpr (STALE) {
+                            if (check | edge_lanes) {
+#pragma unroll
+                                for (int u = 0; u < U; ++u)
+                                    if (hit[u] && !in_current_stencil(g, ps, q[u])) hit[u] = 0;
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            my[cnt][lane] = j + u;
+                            cnt += hit[u];
+                        }
+                        j += U;
+                        if (q[U - 1].z > zhi) je = min(je, j);
+                    }
+                }
+                // the chunk's slots are rewritten only after every lane's reads
+                __builtin_amdgcn_wave_barrier();
+            }
+        } else
         while (true) {
             while (__ballot(j < je) && !__ballot(cnt > CAP - U)) {
                 // a lane whose column is done issues no loads (module calls
@@ -976,11 +1058,13 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
 // UU candidates per load round, WPB waves per workgroup (lists are per wave,
 // so a workgroup holds WPB x CAP x 64 entries of LDS).
 template <class DP, int B, bool MASKED = false, bool ART = false, bool STALE = false, int CAP = IC3_CAP,
-          int UU = IC2_U, int WPB = 4, int PF = 0, int ORD = 0, int LPP = 1>
+          int UU = IC2_U, int WPB = 4, int PF = 0, int ORD = 0, int LPP = 1, bool STG = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(STALE ? 4 : 1))) void
 k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ beg, const int32_t* __restrict__ end,
                  IndexMap mats, int n, int32_t* __restrict__ nbr, StaleIn sn = {}) {
     __shared__ int32_t lists[WPB][CAP][64];
+    // STG: each wave's chunk of staged candidates
+    __shared__ float4 staged[STG ? WPB : 1][STG ? STG_SLOTS : 1];
     // LPP lanes per particle: adjacent lanes of one wave
     const int t = xcd_thread() / LPP;
     const int k = t < n ? mats(t) : -1;
@@ -1031,18 +1115,19 @@ k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ be
         }
     };
     if constexpr (B == 1) {
-        count = scan_compact<1, DP::padded, CAP, STALE, UU, PF, ORD, LPP>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+        count = scan_compact<1, DP::padded, CAP, STALE, UU, PF, ORD, LPP, STG>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
                                             mat ? (uint32_t)k : 0u,
-                                            [&](uint32_t j, bool act) { pair_compute<ART>(pi, load(j, act), c, a); });
+                                            [&](uint32_t j, bool act) { pair_compute<ART>(pi, load(j, act), c, a); },
+                                            staged[STG ? w : 0]);
     } else {
-        count = scan_compact<2, DP::padded, CAP, STALE, UU, PF, ORD, LPP>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+        count = scan_compact<2, DP::padded, CAP, STALE, UU, PF, ORD, LPP, STG>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
                                             mat ? (uint32_t)k : 0u,
                                             [&](uint32_t j0, bool a0, uint32_t j1, bool a1) {
                                                 const PairIn q0 = load(j0, a0);
                                                 const PairIn q1 = load(j1, a1);
                                                 pair_compute<ART>(pi, q0, c, a);
                                                 pair_compute<ART>(pi, q1, c, a);
-                                            });
+                                            }, staged[STG ? w : 0]);
     }
     if constexpr (LPP == 2) {
         // the particle's two partial sums: each lane adds its neighbour's
@@ -1775,8 +1860,8 @@ static void launch_adami(const DP& dp, const DevState& st, const int32_t* beg, c
                          int n, int variant, const StaleIn* stale, int32_t* nbr, hipStream_t s) {
     if (n <= 0) return;
     if (variant >= 12) {
-        // with the variants' partial flushes and edge-first column order, one
-        // wave per workgroup
+        // variants 12 to 17: partial flushes, the edge-first column order
+        // and one wave per workgroup
         if (stale)
             k_adami_lds<DP, true, 1, 16, 2><<<blocks_for(n, 64), 64, 0, s>>>(dp, st, beg, end, walls, n, *stale, nbr);
         else
@@ -1802,20 +1887,20 @@ void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState&
 
 // Variant 11's shape (64-thread workgroups, 40-entry lists, four candidates
 // per round) with partial flushes of PF entries and column order ORD.
-template <int PF, int ORD, int LPP = 1, class DP>
+template <int PF, int ORD, int LPP = 1, int P = IC3_CAP, bool STG = false, class DP>
 static void launch_v11_shape(const DP& dp, const DevState& st, const DevRates& rt, const int32_t* beg,
                              const int32_t* end, IndexMap mats, int n, int32_t* nbr, bool art, const StaleIn* stale,
                              hipStream_t s) {
-    constexpr int P = IC3_CAP, U = IC2_U;
+    constexpr int U = IC2_U;
     const int nb = blocks_for((int64_t)LPP * n, 64);
     if (stale && art)
-        k_interaction_v3<DP, 2, true, true, true, P, U, 1, PF, ORD, LPP><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
+        k_interaction_v3<DP, 2, true, true, true, P, U, 1, PF, ORD, LPP, STG><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
     else if (stale)
-        k_interaction_v3<DP, 2, true, false, true, P, U, 1, PF, ORD, LPP><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
+        k_interaction_v3<DP, 2, true, false, true, P, U, 1, PF, ORD, LPP, STG><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
     else if (art)
-        k_interaction_v3<DP, 2, true, true, false, P, U, 1, PF, ORD, LPP><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+        k_interaction_v3<DP, 2, true, true, false, P, U, 1, PF, ORD, LPP, STG><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else
-        k_interaction_v3<DP, 2, true, false, false, P, U, 1, PF, ORD, LPP><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+        k_interaction_v3<DP, 2, true, false, false, P, U, 1, PF, ORD, LPP, STG><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
 }
 
 template <class DP>
@@ -1824,15 +1909,18 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
                                const StaleIn* stale, hipStream_t s) {
     if (n <= 0) return;
     const int nb = blocks_for(n, 256);
-    // variants 12 to 16: variant 11 with partial flushes of PF entries and
+    // variants 12 to 17: variant 11 with partial flushes of PF entries and
     // the stencil columns in order ORD (see scan_compact); 16 as 13 with two
-    // lanes per particle (32 particles per wave)
+    // lanes per particle (32 particles per wave); 17 as 13 with the
+    // candidates staged in LDS (32-entry lists, to keep sixteen workgroups
+    // per CU)
     switch (variant) {
         case 12: return launch_v11_shape<12, 1>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         case 13: return launch_v11_shape<16, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         case 14: return launch_v11_shape<8, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         case 15: return launch_v11_shape<0, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         case 16: return launch_v11_shape<16, 2, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
+        case 17: return launch_v11_shape<16, 2, 1, 32, true>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         default: break;
     }
     // stale tables: variant 6's shape, in the workgroup size of variants 10
