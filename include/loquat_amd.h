@@ -450,11 +450,13 @@ int32_t lq_engine_get_sort_interval(const lq_engine* eng);
 lq_status lq_engine_profile(lq_engine* eng, int32_t nsteps, double* ms_stage5);
 
 /* Autotuner: times nrep launches of each interaction variant on the engine's
- * resident state and pins the fastest for this engine; the engine's variant
- * stays unless another is more than 1 % faster.  All variants produce the
- * same neighbour sets; their sum orders differ only as stated below.  ms_per_variant
- * (nullable, LQ_INTERACTION_VARIANTS entries) receives the mean launch time
- * of each variant. */
+ * resident state, on the fresh table of a sort and on the kept table (with
+ * each run's recorded displacement bound), weights them as a step does (one
+ * fresh launch in 2K for sort interval K), and pins the fastest for this
+ * engine; the engine's variant stays unless another is more than 1 % faster.
+ * All variants produce the same neighbour sets; their sum orders differ only
+ * as stated below.  ms_per_variant (nullable, LQ_INTERACTION_VARIANTS
+ * entries) receives each variant's weighted mean launch time. */
 #define LQ_INTERACTION_VARIANTS 18
 #define LQ_DEFAULT_INTERACTION_VARIANT 13
 lq_status lq_engine_autotune(lq_engine* eng, int32_t nrep, int32_t* best, double* ms_per_variant);

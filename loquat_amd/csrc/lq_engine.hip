@@ -1195,21 +1195,36 @@ extern "C" lq_status lq_engine_autotune(lq_engine* e, int32_t nrep, int32_t* bes
     LQ_HIP_TRY(hipEventCreate(&ev[0]));
     LQ_HIP_TRY(hipEventCreate(&ev[1]));
     const IndexMap mats{b.mat_idx, 0};
-    double best_ms = 1e30, start_ms = -1.0;
-    int best_v = 3;
-    const int start = e->variant;
-    for (int v = 0; v < LQ_INTERACTION_VARIANTS; ++v) {
+    // A run's steps launch the interaction 2K times per sort interval of K
+    // steps: once on the fresh table of its sort, 2K - 1 times on the kept
+    // table.  Both are timed (the kept-table scans with each run's recorded
+    // displacement bound) and weighted so.
+    StaleIn kept{};
+    kept.po = b.S[e->cur].xo;
+    kept.from_desc = 2;
+    const double w_fresh = 1.0 / (2.0 * e->sort_every), w_kept = 1.0 - w_fresh;
+    auto time_launches = [&](int v, const StaleIn* st, double& ms_out) -> lq_status {
         launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
-                                 (int)e->nmat_total, (int)e->nmat_art, nullptr, v, nullptr, s);  // warm-up
+                                 (int)e->nmat_total, (int)e->nmat_art, nullptr, v, st, s);  // warm-up
         LQ_HIP_TRY(hipEventRecord(ev[0], s));
         for (int r = 0; r < nrep; ++r)
             launch_interaction_table(b.desc, b.run_of, b.S[e->cur], b.R, b.grid.cell_beg, b.grid.cell_end, mats,
-                                     (int)e->nmat_total, (int)e->nmat_art, nullptr, v, nullptr, s);
+                                     (int)e->nmat_total, (int)e->nmat_art, nullptr, v, st, s);
         LQ_HIP_TRY(hipEventRecord(ev[1], s));
         LQ_HIP_TRY(hipEventSynchronize(ev[1]));
         float ms = 0.f;
         LQ_HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
-        const double per = ms / nrep;
+        ms_out = ms / nrep;
+        return LQ_OK;
+    };
+    double best_ms = 1e30, start_ms = -1.0;
+    int best_v = 3;
+    const int start = e->variant;
+    for (int v = 0; v < LQ_INTERACTION_VARIANTS; ++v) {
+        double t_fresh = 0.0, t_kept = 0.0;
+        LQ_TRY(time_launches(v, nullptr, t_fresh));
+        if (e->sort_every > 1) LQ_TRY(time_launches(v, &kept, t_kept));
+        const double per = e->sort_every > 1 ? w_fresh * t_fresh + w_kept * t_kept : t_fresh;
         if (ms_per_variant) ms_per_variant[v] = per;
         if (v == start) start_ms = per;
         if (per < best_ms) {
@@ -1221,9 +1236,8 @@ extern "C" lq_status lq_engine_autotune(lq_engine* e, int32_t nrep, int32_t* bes
     (void)hipEventDestroy(ev[1]);
     LQ_TRY(last_launch("lq_engine_autotune"));
     // the engine's variant stays unless another is clearly faster (1 %,
-    // above the timing noise): the tuner times fresh-table launches, while
-    // most of a run's launches scan kept tables, where near-equal variants
-    // need not keep their order
+    // above the timing noise), so that calls do not flip between near-equal
+    // variants
     if (start_ms > 0.0 && start_ms <= 1.01 * best_ms) best_v = start;
     e->variant = best_v;
     drop_graphs(e);
