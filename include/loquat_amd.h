@@ -398,9 +398,11 @@ lq_status lq_engine_set_cell_order(lq_engine* eng, int32_t zbin_bits);
  * cover a patch T rows wide instead of a single row, and the neighbour
  * gathers of a patch share more of the XCD's L2.  The z cells of a column
  * stay consecutive, cell ids, cell tables and neighbour sets are unchanged,
- * and so is every result bit (each particle's pairs are summed in the same
- * order).  Re-sorts the resident runs.  0 .. LQ_MAX_COLUMN_STRIP; default
- * LQ_DEFAULT_COLUMN_STRIP. */
+ * and each particle's pairs are summed in the same order, so a run gives the
+ * same bits under every column order.  Re-sorts the resident runs (mid-run
+ * that re-sort, like any extra one, moves particles within their cells to
+ * the z-bin order of their current positions: the FP32 sums change order).
+ * 0 .. LQ_MAX_COLUMN_STRIP; default LQ_DEFAULT_COLUMN_STRIP. */
 #define LQ_DEFAULT_COLUMN_STRIP 8
 #define LQ_MAX_COLUMN_STRIP 64
 lq_status lq_engine_set_column_order(lq_engine* eng, int32_t strip_rows);

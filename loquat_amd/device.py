@@ -278,7 +278,8 @@ class Engine:
 
     def set_column_order(self, strip_rows: int) -> None:
         """(x, y) column order of the engine's sort: strips of strip_rows x-rows
-        (0 or 1: cell id order); results are unchanged bit for bit."""
+        (0 or 1: cell id order).  A run gives the same bits under every order;
+        the re-sort this call makes mid-run changes the order of FP32 sums."""
         abi.check(abi.lib().lq_engine_set_column_order(self.h, int(strip_rows)))
 
     def column_order(self) -> int:
