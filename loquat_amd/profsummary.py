@@ -139,7 +139,9 @@ def main() -> None:
                   f"{w_kib * wf * 1024 / 1e6:.1f} MB |")
     sqc = ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS", "SQ_ACTIVE_INST_VALU",
            "SQ_BUSY_CU_CYCLES", "SQ_WAIT_ANY", "SQ_WAVE_CYCLES"]
-    tac = ["GRBM_GUI_ACTIVE", "TA_TA_BUSY", "TD_TD_BUSY", "TCP_TOTAL_CACHE_ACCESSES"]
+    tac = ["GRBM_GUI_ACTIVE", "TA_TA_BUSY", "TA_ADDR_STALLED_BY_TC_CYCLES", "TD_TD_BUSY", "TD_TC_STALL",
+           "TCP_TOTAL_CACHE_ACCESSES", "TCP_PENDING_STALL_CYCLES"]
+    tac = [c for c in tac if any(c in ta[k] for k in matched)]
     md += ["", "## Instruction and memory-pipeline counters per dispatch (interaction launches)", "",
            "| kernel | " + " | ".join(sqc) + " |", "|---" * (len(sqc) + 1) + "|"]
     for k in matched:
