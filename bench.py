@@ -37,33 +37,39 @@ BYTES_READ_PER_PARTICLE = 64   # x (FP64, 24) + v (12) + rho (4) + stress (24)
 BYTES_WRITE_PER_MATERIAL = 52  # a (12) + drho (4) + spin (12) + strain rate (24)
 BYTES_PER_CELL = 8             # cell_beg + cell_end
 WORKLOAD_ID = "column_3d dx=5mm npm=1024000"  # the key of this workload in profiles/*.json
+# Template arguments of k_interaction_v3 per variant (3 and up): pairs per
+# iteration, masked gathers, list capacity, candidates per round, waves per
+# workgroup, partial-flush size, column order, lanes per particle, staging.
+V3_SHAPES = {3: (1, False, 40, 4, 4, 0, 0, 1, False), 4: (2, False, 40, 4, 4, 0, 0, 1, False),
+             5: (1, True, 40, 4, 4, 0, 0, 1, False), 6: (2, True, 40, 4, 4, 0, 0, 1, False),
+             7: (2, True, 32, 4, 4, 0, 0, 1, False), 8: (2, True, 48, 4, 4, 0, 0, 1, False),
+             9: (2, True, 40, 8, 4, 0, 0, 1, False), 10: (2, True, 40, 4, 2, 0, 0, 1, False),
+             11: (2, True, 40, 4, 1, 0, 0, 1, False), 12: (2, True, 40, 4, 1, 12, 1, 1, False),
+             13: (2, True, 40, 4, 1, 16, 2, 1, False), 14: (2, True, 40, 4, 1, 8, 2, 1, False),
+             15: (2, True, 40, 4, 1, 0, 2, 1, False), 16: (2, True, 40, 4, 1, 16, 2, 2, False),
+             17: (2, True, 32, 4, 1, 16, 2, 1, True), 18: (2, False, 40, 4, 1, 16, 2, 1, False),
+             19: (2, False, 32, 4, 1, 16, 2, 1, True)}
+
+
+def v3_name(shape: tuple, stale: bool) -> str:
+    b, masked, cap, u, wpb, pf, order, lpp, stg = shape
+    f = lambda x: "true" if x else "false"  # noqa: E731
+    return (f"k_interaction_v3<DescTable, {b}, {f(masked)}, false, {f(stale)}, {cap}, {u}, {wpb}, {pf}, {order}, "
+            f"{lpp}, {f(stg)}>")
+
+
 KERNEL_NAMES = {0: "k_interaction_direct", 1: "k_interaction_lds", 2: "k_interaction_unrolled",
-                3: "k_interaction_v3<DescTable, 1, false, false, false, 40, 4, 4, 0, 0, 1, false>",
-                4: "k_interaction_v3<DescTable, 2, false, false, false, 40, 4, 4, 0, 0, 1, false>",
-                5: "k_interaction_v3<DescTable, 1, true, false, false, 40, 4, 4, 0, 0, 1, false>",
-                6: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 4, 0, 0, 1, false>",
-                7: "k_interaction_v3<DescTable, 2, true, false, false, 32, 4, 4, 0, 0, 1, false>",
-                8: "k_interaction_v3<DescTable, 2, true, false, false, 48, 4, 4, 0, 0, 1, false>",
-                9: "k_interaction_v3<DescTable, 2, true, false, false, 40, 8, 4, 0, 0, 1, false>",
-                10: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 2, 0, 0, 1, false>",
-                11: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 0, 0, 1, false>",
-                12: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 12, 1, 1, false>",
-                13: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 16, 2, 1, false>",
-                14: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 8, 2, 1, false>",
-                15: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 0, 2, 1, false>",
-                16: "k_interaction_v3<DescTable, 2, true, false, false, 40, 4, 1, 16, 2, 2, false>",
-                17: "k_interaction_v3<DescTable, 2, true, false, false, 32, 4, 1, 16, 2, 1, true>"}
-# Between a run's sorts both half steps scan its kept cell table with variant
-# 6's shape, in the workgroup size of variants 10 (128 threads) and 11 to 17
-# (64 threads; 12 to 17 with their partial flushes and column orders, 16 with
-# two lanes per particle, 17 with staged candidates); the predictor half of a
-# sort step scans the fresh table with the variant.
+                **{v: v3_name(sh, False) for v, sh in V3_SHAPES.items()}}
+
+
+# Between a run's sorts both half steps scan its kept cell table: variants 11
+# and up in their own shape, the others in variant 6's shape (variant 10's
+# workgroup size for 10); the predictor half of a sort step scans the fresh
+# table with the variant.
 def stale_kernel(variant: int) -> str:
     if variant >= 11:
-        return KERNEL_NAMES[variant].replace("false, false, false", "false, false, true").replace(
-            "true, false, false", "true, false, true")
-    wpb = {10: 2}.get(variant, 4)
-    return f"k_interaction_v3<DescTable, 2, true, false, true, 40, 4, {wpb}, 0, 0, 1, false>"
+        return v3_name(V3_SHAPES[variant], True)
+    return v3_name((2, True, 40, 4, 2 if variant == 10 else 4, 0, 0, 1, False), True)
 
 
 def roofline_kernel(variant: int) -> str:

@@ -459,8 +459,8 @@ lq_status lq_engine_profile(lq_engine* eng, int32_t nsteps, double* ms_stage5);
  * All variants produce the same neighbour sets; their sum orders differ only
  * as stated below.  ms_per_variant (nullable, LQ_INTERACTION_VARIANTS
  * entries) receives each variant's weighted mean launch time. */
-#define LQ_INTERACTION_VARIANTS 18
-#define LQ_DEFAULT_INTERACTION_VARIANT 13
+#define LQ_INTERACTION_VARIANTS 20
+#define LQ_DEFAULT_INTERACTION_VARIANT 19
 lq_status lq_engine_autotune(lq_engine* eng, int32_t nrep, int32_t* best, double* ms_per_variant);
 
 /* Picks the in-cell order (lq_engine_set_cell_order: 2^b height bins per
@@ -499,21 +499,24 @@ lq_status lq_engine_autotune_cell_order(lq_engine* eng, int32_t nrep, int32_t* b
  *        order that keeps the lanes' list lengths alike (opposite columns in
  *        pairs): 12 PF = 12 with opposite columns pairwise, own column last;
  *        13 PF = 16 with the edge columns first, then the corners, own
- *        column last (the default); 14 as 13 with PF = 8; 15 as 13 with full
- *        flushes,
+ *        column last; 14 as 13 with PF = 8; 15 as 13 with full flushes,
  * 16 = 13 with two lanes per particle (32 particles per wave, the lane pair
  *        taking a column's candidates alternately, the two partial sums
  *        added at the end),
  * 17 = 13 with 32-entry lists and the candidates of each stencil column
  *        staged in LDS by the wave (chunks of 128 consecutive particles from
  *        its lanes' lowest pending candidate, loaded coalesced; chunk bounds
- *        by wavefront-shuffle minimum / maximum) and tested from there.
+ *        by wavefront-shuffle minimum / maximum) and tested from there,
+ * 18, 19 = 13 and 17 with unmasked gathers (a lane past the end of its list
+ *        re-reads its own particle, whose pair terms are exactly zero,
+ *        instead of substituting it from registers); 19 is the default.
  *        Variants 3 - 11 sum each particle's pairs in one order, 13 - 15 and
- *        17 in another, 12 and 16 in their own (all within FP32 rounding).
+ *        17 - 19 in another, 12 and 16 in their own (all within FP32
+ *        rounding).
  * Variants 3 and up also select the list-based Adami and Shepard scans; the
  * scans of a kept (stale) cell table and the artificial-pressure kernel use
- * variant 6's shape (in the workgroup size of variants 10 to 17, with their
- * flushes, column orders, lanes per particle and staging). */
+ * variant 6's shape (in the workgroup size of variants 10 to 19, with their
+ * flushes, column orders, lanes per particle, staging and gathers). */
 lq_status lq_set_interaction_variant(int32_t variant);
 int32_t lq_get_interaction_variant(void);
 lq_status lq_engine_set_interaction_variant(lq_engine* eng, int32_t variant);

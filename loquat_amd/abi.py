@@ -86,9 +86,9 @@ class LqServerConfig(C.Structure):
 
 
 REQ_STATE = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "rejected"}
-INTERACTION_VARIANTS = 18  # LQ_INTERACTION_VARIANTS
+INTERACTION_VARIANTS = 20  # LQ_INTERACTION_VARIANTS
 CELL_ORDER_CANDIDATES = 6  # LQ_CELL_ORDER_CANDIDATES
-DEFAULT_INTERACTION_VARIANT = 13  # LQ_DEFAULT_INTERACTION_VARIANT
+DEFAULT_INTERACTION_VARIANT = 19  # LQ_DEFAULT_INTERACTION_VARIANT
 COLUMN_ORDER_CANDIDATES = 5  # LQ_COLUMN_ORDER_CANDIDATES
 DEFAULT_SORT_INTERVAL = 8  # LQ_DEFAULT_SORT_INTERVAL
 MAX_SORT_INTERVAL = 16  # LQ_MAX_SORT_INTERVAL

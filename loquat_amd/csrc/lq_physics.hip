@@ -1053,13 +1053,14 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
 // no sum, and one is taken off the neighbour count.  Variant 4 (B = 2)
 // evaluates the lists two pairs at a time; same pairs, same order.  ART adds
 // the artificial pressure; STALE scans a table built before the particles
-// last moved (sn), held to four waves per SIMD (at most 128 VGPRs).
+// last moved (sn).  Kernels without artificial pressure are held to four
+// waves per SIMD (at most 128 VGPRs).
 // Shape parameters searched by the autotuner: CAP list entries per lane,
 // UU candidates per load round, WPB waves per workgroup (lists are per wave,
 // so a workgroup holds WPB x CAP x 64 entries of LDS).
 template <class DP, int B, bool MASKED = false, bool ART = false, bool STALE = false, int CAP = IC3_CAP,
           int UU = IC2_U, int WPB = 4, int PF = 0, int ORD = 0, int LPP = 1, bool STG = false>
-__global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(STALE ? 4 : 1))) void
+__global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu((STALE || !ART) ? 4 : 1))) void
 k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ beg, const int32_t* __restrict__ end,
                  IndexMap mats, int n, int32_t* __restrict__ nbr, StaleIn sn = {}) {
     __shared__ int32_t lists[WPB][CAP][64];
@@ -1860,7 +1861,7 @@ static void launch_adami(const DP& dp, const DevState& st, const int32_t* beg, c
                          int n, int variant, const StaleIn* stale, int32_t* nbr, hipStream_t s) {
     if (n <= 0) return;
     if (variant >= 12) {
-        // variants 12 to 17: partial flushes, the edge-first column order
+        // variants 12 to 19: partial flushes, the edge-first column order
         // and one wave per workgroup
         if (stale)
             k_adami_lds<DP, true, 1, 16, 2><<<blocks_for(n, 64), 64, 0, s>>>(dp, st, beg, end, walls, n, *stale, nbr);
@@ -1887,20 +1888,20 @@ void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState&
 
 // Variant 11's shape (64-thread workgroups, 40-entry lists, four candidates
 // per round) with partial flushes of PF entries and column order ORD.
-template <int PF, int ORD, int LPP = 1, int P = IC3_CAP, bool STG = false, class DP>
+template <int PF, int ORD, int LPP = 1, int P = IC3_CAP, bool STG = false, bool M = true, class DP>
 static void launch_v11_shape(const DP& dp, const DevState& st, const DevRates& rt, const int32_t* beg,
                              const int32_t* end, IndexMap mats, int n, int32_t* nbr, bool art, const StaleIn* stale,
                              hipStream_t s) {
     constexpr int U = IC2_U;
     const int nb = blocks_for((int64_t)LPP * n, 64);
     if (stale && art)
-        k_interaction_v3<DP, 2, true, true, true, P, U, 1, PF, ORD, LPP, STG><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
+        k_interaction_v3<DP, 2, M, true, true, P, U, 1, PF, ORD, LPP, STG><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
     else if (stale)
-        k_interaction_v3<DP, 2, true, false, true, P, U, 1, PF, ORD, LPP, STG><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
+        k_interaction_v3<DP, 2, M, false, true, P, U, 1, PF, ORD, LPP, STG><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
     else if (art)
-        k_interaction_v3<DP, 2, true, true, false, P, U, 1, PF, ORD, LPP, STG><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+        k_interaction_v3<DP, 2, M, true, false, P, U, 1, PF, ORD, LPP, STG><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
     else
-        k_interaction_v3<DP, 2, true, false, false, P, U, 1, PF, ORD, LPP, STG><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
+        k_interaction_v3<DP, 2, M, false, false, P, U, 1, PF, ORD, LPP, STG><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr);
 }
 
 template <class DP>
@@ -1909,11 +1910,11 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
                                const StaleIn* stale, hipStream_t s) {
     if (n <= 0) return;
     const int nb = blocks_for(n, 256);
-    // variants 12 to 17: variant 11 with partial flushes of PF entries and
+    // variants 12 to 19: variant 11 with partial flushes of PF entries and
     // the stencil columns in order ORD (see scan_compact); 16 as 13 with two
     // lanes per particle (32 particles per wave); 17 as 13 with the
     // candidates staged in LDS (32-entry lists, to keep sixteen workgroups
-    // per CU)
+    // per CU); 18 and 19 as 13 and 17 with unmasked gathers
     switch (variant) {
         case 12: return launch_v11_shape<12, 1>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         case 13: return launch_v11_shape<16, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
@@ -1921,6 +1922,8 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
         case 15: return launch_v11_shape<0, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         case 16: return launch_v11_shape<16, 2, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         case 17: return launch_v11_shape<16, 2, 1, 32, true>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
+        case 18: return launch_v11_shape<16, 2, 1, IC3_CAP, false, false>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
+        case 19: return launch_v11_shape<16, 2, 1, 32, true, false>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         default: break;
     }
     // stale tables: variant 6's shape, in the workgroup size of variants 10

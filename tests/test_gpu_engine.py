@@ -167,12 +167,12 @@ def test_interaction_variants_agree_in_engine(gpu):
             assert rel_err(res[0][k], r[k]) < 1e-5
     # variants 3 to 11 run the same pair code on the same pairs in the same
     # order; 12 visits the stencil columns in another order, 13 to 15 and 17
-    # in a third (partial flushes and staging keep the order); 16 splits the
-    # sums over two lanes
+    # to 19 in a third (partial flushes, staging and masking keep the order);
+    # 16 splits the sums over two lanes
     by_v = dict(zip(vs, res))
     for v in range(4, 12):
         assert _same(by_v[3], by_v[v]), v
-    for v in (14, 15, 17):
+    for v in (14, 15, 17, 18, 19):
         assert _same(by_v[13], by_v[v]), v
 
 
