@@ -947,17 +947,19 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                const int lim = base + STG_SLOTS;
+                // a block is tested from this chunk when its candidates below
+                // je end within the first STG_SLOTS - U slots: its four reads
+                // (j - base + 3 < STG_SLOTS) then stay in the chunk
+                const int lim = base + (STG_SLOTS - U);
                 while (true) {
                     if (__ballot(cnt > CAP - U)) flush_full();
                     const bool can = j < je && min(j + U, je) <= lim;
                     if (!__ballot(can)) break;
                     if (can) {
-                        // slots past the chunk hold candidates past je (not
-                        // hits): the index is clamped into the chunk
                         float4 q[U];
+                        const float4* sq = stage + (j - base);
 #pragma unroll
-                        for (int u = 0; u < U; ++u) q[u] = stage[min(j - base + u, STG_SLOTS - 1)];
+                        for (int u = 0; u < U; ++u) q[u] = sq[u];
                         int hit[U];
                         unsigned long long check = 0ull;
 #pragma unroll
