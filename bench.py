@@ -483,6 +483,10 @@ def main() -> None:
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if a.variant is None:
+        a.variant = abi.DEFAULT_INTERACTION_VARIANT
+    # the calling thread's variant: engines created from here on start with it
+    abi.check(abi.lib().lq_set_interaction_variant(a.variant))
     if a.slope_only:
         sec = slope_section(a.slope_steps, autotune=bool(a.autotune), cell_order=a.cell_order)
         print(json.dumps({"slope_vw": sec}), flush=True)
@@ -494,9 +498,6 @@ def main() -> None:
     if a.served_only:
         print(json.dumps({"served": served_section(a.serve_requests, a.serve_step_scale)}), flush=True)
         return
-    if a.variant is None:
-        a.variant = abi.DEFAULT_INTERACTION_VARIANT
-    abi.check(abi.lib().lq_set_interaction_variant(a.variant))
 
     # room for the timed steps twice: in the tuned order and in the reference order
     case = cases.column_3d(nsteps=2 * (a.warmup + a.steps + a.profile_steps) + 1)

@@ -509,7 +509,10 @@ lq_status lq_engine_autotune_cell_order(lq_engine* eng, int32_t nrep, int32_t* b
  *        by wavefront-shuffle minimum / maximum) and tested from there,
  * 18, 19 = 13 and 17 with unmasked gathers (a lane past the end of its list
  *        re-reads its own particle, whose pair terms are exactly zero,
- *        instead of substituting it from registers); 19 is the default.
+ *        instead of substituting it from registers); 19 stages only
+ *        launches of at least 524,288 material particles and runs as 18
+ *        below (the same bits either way: small launches are latency-bound,
+ *        where staging costs); 19 is the default.
  *        Variants 3 - 11 sum each particle's pairs in one order, 13 - 15 and
  *        17 - 19 in another, 12 and 16 in their own (all within FP32
  *        rounding).

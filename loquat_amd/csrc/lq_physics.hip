@@ -55,6 +55,11 @@ __device__ __forceinline__ int wave_max_i32(int v) {
 // allow (with 4 more slots, or 40-entry lists, fifteen or thirteen fit, and
 // the launch slows by 9 % and 15 %).
 constexpr int STG_SLOTS = 128;
+// Variant 19 stages the scan from this many material particles per launch
+// (single columns, ms per step, variant 18 unstaged / 19 staged: 0.288 /
+// 0.339 at 65k, 0.403 / 0.423 at 221k, 0.726 / 0.721 at 524k, 1.264 / 1.211
+// at 1.02 M).
+constexpr int STG_MIN_LAUNCH = 1 << 19;
 
 // The value of lane ^ 1 (DPP quad permutation 1, 0, 3, 2).
 __device__ __forceinline__ float swap_pair(float x) {
@@ -1916,7 +1921,8 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
     // the stencil columns in order ORD (see scan_compact); 16 as 13 with two
     // lanes per particle (32 particles per wave); 17 as 13 with the
     // candidates staged in LDS (32-entry lists, to keep sixteen workgroups
-    // per CU); 18 and 19 as 13 and 17 with unmasked gathers
+    // per CU); 18 and 19 as 13 and 17 with unmasked gathers, 19 staged only
+    // on launches of at least STG_MIN_LAUNCH particles
     switch (variant) {
         case 12: return launch_v11_shape<12, 1>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         case 13: return launch_v11_shape<16, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
@@ -1925,7 +1931,13 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
         case 16: return launch_v11_shape<16, 2, 2>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         case 17: return launch_v11_shape<16, 2, 1, 32, true>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         case 18: return launch_v11_shape<16, 2, 1, IC3_CAP, false, false>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
-        case 19: return launch_v11_shape<16, 2, 1, 32, true, false>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
+        case 19:
+            // 18 and 19 give the same bits: staged only where it pays, on
+            // launches large enough to fill the GPU many times over (a small
+            // launch's time is one wave's latency, which staging lengthens)
+            if (n >= STG_MIN_LAUNCH)
+                return launch_v11_shape<16, 2, 1, 32, true, false>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
+            return launch_v11_shape<16, 2, 1, IC3_CAP, false, false>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
         default: break;
     }
     // stale tables: variant 6's shape, in the workgroup size of variants 10
