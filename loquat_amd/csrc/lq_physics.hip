@@ -952,13 +952,13 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                // a block is tested from this chunk when its candidates below
-                // je end within the first STG_SLOTS - U slots: its four reads
+                // a block is tested from this chunk when it starts within the
+                // first STG_SLOTS - U + 1 slots: its four reads
                 // (j - base + 3 < STG_SLOTS) then stay in the chunk
                 const int lim = base + (STG_SLOTS - U);
                 while (true) {
                     if (__ballot(cnt > CAP - U)) flush_full();
-                    const bool can = j < je && min(j + U, je) <= lim;
+                    const bool can = j < je && j <= lim;
                     if (!__ballot(can)) break;
                     if (can) {
                         float4 q[U];
@@ -966,15 +966,16 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
 #pragma unroll
                         for (int u = 0; u < U; ++u) q[u] = sq[u];
                         int hit[U];
-                        unsigned long long check = 0ull;
+                        bool edge = false;  // a hit past ext2: the exact cell test decides
 #pragma unroll
                         for (int u = 0; u < U; ++u) {
                             const float r2 = r2_fp32(ps.x - q[u].x, ps.y - q[u].y, ps.z - q[u].z);
-                            hit[u] = (r2 < r2cut ? 1 : 0) & ((j + u < je) ? 1 : 0);
-                            if constexpr (STALE) check |= __ballot(r2 >= g.ext2 && r2 < r2cut);
+                            const bool in = r2 < r2cut;
+                            hit[u] = (in && (u == 0 || j + u < je)) ? 1 : 0;  // j < je holds
+                            if constexpr (STALE) edge = edge | (in & (r2 >= g.ext2));
                         }
                         if constexpr (STALE) {
-                            if (check | edge_lanes) {
+                            if (__ballot(edge) | edge_lanes) {
 #pragma unroll
                                 for (int u = 0; u < U; ++u)
                                     if (hit[u] && !in_current_stencil(g, ps, q[u])) hit[u] = 0;
@@ -1010,23 +1011,24 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
                     for (int u = 0; u < U; ++u) q[u] = ldo(pp, min((uint32_t)(j + sub + LPP * u), jlast));
                 }
                 int hit[U];
-                unsigned long long check = 0ull;
+                bool edge = false;
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     // both tests evaluated, combined without a branch
                     const float r2 = r2_fp32(ps.x - q[u].x, ps.y - q[u].y, ps.z - q[u].z);
-                    const int close = r2 < r2cut ? 1 : 0;
-                    const int inside = (j + sub + LPP * u < je) ? 1 : 0;
-                    hit[u] = close & inside;
+                    const bool in = r2 < r2cut;
+                    hit[u] = (in & (j + sub + LPP * u < je)) ? 1 : 0;
                     // Closer than (1 - eps) cell (so along every axis), two
                     // particles' cells differ by at most one (the FP32
                     // rounding of their cell coordinates is well below eps),
                     // so only hits at that distance or of a lane on the
-                    // grid's edge need the exact cell test.
-                    if constexpr (STALE) check |= __ballot(r2 >= g.ext2 && r2 < r2cut);
+                    // grid's edge need the exact cell test.  (Bitwise, not
+                    // short-circuit: the lane masks combine in scalar
+                    // registers.)
+                    if constexpr (STALE) edge = edge | (in & (r2 >= g.ext2));
                 }
                 if constexpr (STALE) {
-                    if (check | edge_lanes) {
+                    if (__ballot(edge) | edge_lanes) {
 #pragma unroll
                         for (int u = 0; u < U; ++u)
                             if (hit[u] && !in_current_stencil(g, ps, q[u])) hit[u] = 0;
