@@ -954,24 +954,33 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 // a block is tested from this chunk when it starts within the
                 // first STG_SLOTS - U + 1 slots: its four reads
-                // (j - base + 3 < STG_SLOTS) then stay in the chunk
-                const int lim = base + (STG_SLOTS - U);
+                // (j - base + 3 < STG_SLOTS) then stay in the chunk; jlim =
+                // -1 marks a column that ended in this chunk
+                int jlim = min(je, base + (STG_SLOTS - U) + 1);
+                // the chunk slot of j and the list entry my[cnt][lane] as LDS
+                // addresses, advanced in place
+                const float4* sq = stage + (j - base);
+                int32_t* wp = &my[cnt][lane];
+                int32_t* const wfull = &my[CAP - U][lane];
                 while (true) {
-                    if (__ballot(cnt > CAP - U)) flush_full();
-                    const bool can = j < je && j <= lim;
+                    if (__ballot(wp > wfull)) {
+                        cnt = (int)((wp - &my[0][lane]) >> 6);
+                        flush_full();
+                        wp = &my[cnt][lane];
+                    }
+                    const bool can = j < jlim;
                     if (!__ballot(can)) break;
                     if (can) {
                         float4 q[U];
-                        const float4* sq = stage + (j - base);
 #pragma unroll
                         for (int u = 0; u < U; ++u) q[u] = sq[u];
-                        int hit[U];
+                        int hit[U];  // 256 (the byte stride of list entries) or 0
                         bool edge = false;  // a hit past ext2: the exact cell test decides
 #pragma unroll
                         for (int u = 0; u < U; ++u) {
                             const float r2 = r2_fp32(ps.x - q[u].x, ps.y - q[u].y, ps.z - q[u].z);
                             const bool in = r2 < r2cut;
-                            hit[u] = (in && (u == 0 || j + u < je)) ? 1 : 0;  // j < je holds
+                            hit[u] = (in && (u == 0 || j + u < je)) ? 256 : 0;  // j < je holds
                             if constexpr (STALE) edge = edge | (in & (r2 >= g.ext2));
                         }
                         if constexpr (STALE) {
@@ -983,13 +992,16 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
                         }
 #pragma unroll
                         for (int u = 0; u < U; ++u) {
-                            my[cnt][lane] = j + u;
-                            cnt += hit[u];
+                            *wp = j + u;
+                            wp = (int32_t*)((char*)wp + hit[u]);
                         }
                         j += U;
-                        if (q[U - 1].z > zhi) je = min(je, j);
+                        sq += U;
+                        if (q[U - 1].z > zhi) jlim = -1;
                     }
                 }
+                if (jlim < 0) je = min(je, j);
+                cnt = (int)((wp - &my[0][lane]) >> 6);
                 // the chunk's slots are rewritten only after every lane's reads
                 __builtin_amdgcn_wave_barrier();
             }
