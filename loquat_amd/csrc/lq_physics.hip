@@ -1171,8 +1171,11 @@ k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ be
 // Adami wall stress with the list-based scan (material layer only; a wall is
 // never in its own candidate stream).  Same sums as k_adami, same order.
 // WPB waves per workgroup; PF and ORD as in scan_compact (partial flushes,
-// column order).
-template <class DP, bool STALE = false, int WPB = 4, int PF = 0, int ORD = 0>
+// column order).  B = 2 gathers two listed neighbours' data before summing
+// either (the same sums in the same order): a wave of walls next to the
+// material is bound by its chain of dependent gathers, one round per flush
+// iteration.
+template <class DP, bool STALE = false, int WPB = 4, int PF = 0, int ORD = 0, int B = 1, int U = IC2_U>
 __global__ __launch_bounds__(256) void k_adami_lds(DP dp, DevState st, const int32_t* __restrict__ beg,
                                                    const int32_t* __restrict__ end, IndexMap walls, int n,
                                                    StaleIn sn = {}, int32_t* __restrict__ nbr = nullptr) {
@@ -1207,17 +1210,14 @@ __global__ __launch_bounds__(256) void k_adami_lds(DP dp, DevState st, const int
     }
     float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
     float px = 0.f, py = 0.f, pz = 0.f, wsum = 0.f;
-    const int cnt = scan_compact<1, DP::padded, IC2_CAP, STALE, IC2_U, PF, ORD>(on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w],
-                                                                 lane, on ? (uint32_t)k : 0u,
-                             [&](uint32_t j, bool act) {
-        const float dx = (float)(xi - ldo(st.xd, j)), dy = (float)(yi - ldo(st.yd, j)),
-                    dz = (float)(zi - ldo(st.zd, j));
+    // one pair's sums from its gathered data
+    auto add = [&](double xj, double yj, double zj, float rhoj, const tensor2d& s, bool act) {
+        const float dx = (float)(xi - xj), dy = (float)(yi - yj), dz = (float)(zi - zj);
         const float q = fsqrt(dx * dx + dy * dy + dz * dz) * i_h;
         const float tt = fmaxf(2.0f - q, 0.0f);
         const float t2 = tt * tt;
-        const float kv0 = cw * t2 * t2 * (q + 0.5f) * m * frcp(ldo(st.velrhop, j).w);
+        const float kv0 = cw * t2 * t2 * (q + 0.5f) * m * frcp(rhoj);
         const float kv = act ? kv0 : 0.0f;
-        const tensor2d s = ldo(st.str, j);
         sxx += s.xx * kv;
         syy += s.yy * kv;
         szz += s.zz * kv;
@@ -1228,7 +1228,26 @@ __global__ __launch_bounds__(256) void k_adami_lds(DP dp, DevState st, const int
         py += dy * kv;
         pz += dz * kv;
         wsum += kv;
-    });
+    };
+    int cnt;
+    if constexpr (B == 1) {
+        cnt = scan_compact<1, DP::padded, IC2_CAP, STALE, U, PF, ORD>(
+            on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane, on ? (uint32_t)k : 0u,
+            [&](uint32_t j, bool act) {
+                add(ldo(st.xd, j), ldo(st.yd, j), ldo(st.zd, j), ldo(st.velrhop, j).w, ldo(st.str, j), act);
+            });
+    } else {
+        cnt = scan_compact<2, DP::padded, IC2_CAP, STALE, U, PF, ORD>(
+            on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane, on ? (uint32_t)k : 0u,
+            [&](uint32_t j0, bool a0, uint32_t j1, bool a1) {
+                const double x0 = ldo(st.xd, j0), y0 = ldo(st.yd, j0), z0 = ldo(st.zd, j0);
+                const double x1 = ldo(st.xd, j1), y1 = ldo(st.yd, j1), z1 = ldo(st.zd, j1);
+                const float r0 = ldo(st.velrhop, j0).w, r1 = ldo(st.velrhop, j1).w;
+                const tensor2d s0 = ldo(st.str, j0), s1 = ldo(st.str, j1);
+                add(x0, y0, z0, r0, s0, a0);
+                add(x1, y1, z1, r1, s1, a1);
+            });
+    }
     if (on && nbr) nbr[k] = cnt;
     if (on && wsum > 0.0f) {
         const RunDesc& d = *dptr;
@@ -1881,8 +1900,17 @@ template <class DP>
 static void launch_adami(const DP& dp, const DevState& st, const int32_t* beg, const int32_t* end, IndexMap walls,
                          int n, int variant, const StaleIn* stale, int32_t* nbr, hipStream_t s) {
     if (n <= 0) return;
+    if (variant == 19) {
+        // variant 18's shape gathering two pairs at a time (same sums)
+        if (stale)
+            k_adami_lds<DP, true, 1, 16, 2, 2, 8><<<blocks_for(n, 64), 64, 0, s>>>(dp, st, beg, end, walls, n, *stale, nbr);
+        else
+            k_adami_lds<DP, false, 1, 16, 2, 2, 8>
+                <<<blocks_for(n, 64), 64, 0, s>>>(dp, st, beg, end, walls, n, StaleIn{}, nbr);
+        return;
+    }
     if (variant >= 12) {
-        // variants 12 to 19: partial flushes, the edge-first column order
+        // variants 12 to 18: partial flushes, the edge-first column order
         // and one wave per workgroup
         if (stale)
             k_adami_lds<DP, true, 1, 16, 2><<<blocks_for(n, 64), 64, 0, s>>>(dp, st, beg, end, walls, n, *stale, nbr);
