@@ -185,11 +185,12 @@ __device__ __forceinline__ uint32_t zbin_in_cell(float z, float i_h, int zbits, 
     return (uint32_t)fminf(fmaxf(f, 0.0f), nb - 1.0f);
 }
 
-// FP32 squared distance with a fixed operation order and no contraction; the
+// FP32 squared distance with a fixed operation order: dx * dx rounded, then
+// two fused multiply-adds (the same rounding on every implementation); the
 // neighbour predicate is r2 < RunDesc::r2_cut.
 __device__ __forceinline__ float r2_fp32(float dx, float dy, float dz) {
 #pragma clang fp contract(off)
-    return (dx * dx + dy * dy) + dz * dz;
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
 // Calls f(jb, je) for the contiguous particle range of each (x, y) column of

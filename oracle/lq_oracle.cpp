@@ -26,7 +26,7 @@
 // FP32 roundings of the positions, and the cell hash is computed in FP32.
 // Pair terms use the FP64 position differences.
 //
-// Build: make -C oracle (g++ -O2 -fopenmp -ffp-contract=off).
+// Build: make -C oracle (g++ -O2 -fopenmp -ffp-contract=off -mfma).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -100,10 +100,11 @@ inline int cell_coord(float x, float i_h) {
 }
 
 // Neighbour predicate q = r / h < 2 (src/adami_boundary.cu:41-43), evaluated
-// in FP32 on the FP32 position roundings, sqrt correctly rounded.
+// in FP32 on the FP32 position roundings: r2 = fma(dz, dz, fma(dy, dy, dx dx))
+// with dx dx rounded, sqrt correctly rounded.
 inline bool is_nbr(float xi, float yi, float zi, float xj, float yj, float zj, float i_h) {
     const float dx = xi - xj, dy = yi - yj, dz = zi - zj;
-    const float r2 = (dx * dx + dy * dy) + dz * dz;
+    const float r2 = std::fmaf(dz, dz, std::fmaf(dy, dy, dx * dx));
     return sqrtf(r2) * i_h < 2.0f;
 }
 

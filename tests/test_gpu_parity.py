@@ -8,7 +8,7 @@ max_i |gpu - oracle| / max_i |oracle|.
 import numpy as np
 import pytest
 
-from helpers import rel_err
+from helpers import r2_fp32, rel_err
 from loquat_amd import abi, cases
 from oracle import lq_oracle as O
 
@@ -435,7 +435,7 @@ def _brute_counts(p, pos32: np.ndarray, npb: int) -> np.ndarray:
     pairs = cKDTree(pos32.astype(np.float64)).query_pairs(r=1.01 * float(np.sqrt(r2c)), output_type="ndarray")
     i, j = pairs[:, 0], pairs[:, 1]
     d = pos32[i] - pos32[j]
-    r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    r2 = r2_fp32(d)
     near = np.all(np.abs(cells[i] - cells[j]) <= 1, axis=1) & (r2 < r2c)
     mat = np.arange(len(pos32)) >= npb
     either = mat[i] | mat[j]

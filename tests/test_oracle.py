@@ -14,6 +14,7 @@ import pytest
 
 from loquat_amd import abi, cases
 from oracle import lq_oracle as O
+from helpers import fma32, r2_fp32
 
 
 def lattice_case(nx=14, ny=14, nz=14, walls=True):
@@ -84,6 +85,37 @@ def test_sort_is_stable_and_table_brute_force():
             assert beg[c] == -1 and end[c] == -1
 
 
+def _round_f32_exact(v) -> np.float32:
+    """Nearest FP32 to the rational v, ties to even."""
+    from fractions import Fraction
+    r = np.float32(float(v))
+    cands = [np.nextafter(r, np.float32(-np.inf)), r, np.nextafter(r, np.float32(np.inf))]
+    best = min(cands, key=lambda x: (abs(Fraction(float(x)) - v), int(np.float32(x).view(np.uint32)) & 1))
+    return np.float32(best)
+
+
+def test_fma32_is_correctly_rounded():
+    """The tests' FP32 fused multiply-add (the neighbour predicate's r2):
+    (1 + 2^-12)^2 = 1 + 2^-11 + 2^-24 lies halfway between two FP32 values,
+    so the unfused product rounds to even, the fused one keeps the 2^-24, and
+    a term far below FP64's resolution decides a tie the FP64 sum cannot
+    see; random and near-tie triples against exact rational arithmetic."""
+    from fractions import Fraction
+    a = np.float32(1 + 2.0 ** -12)
+    assert np.float32(a * a) - np.float32(1) == np.float32(2.0 ** -11)
+    assert fma32(a, a, np.float32(-1.0)) == np.float32(2.0 ** -11 + 2.0 ** -24)
+    assert fma32(a, a, np.float32(2.0 ** -80)) == np.float32(1 + 2.0 ** -11 + 2.0 ** -23)
+    assert fma32(a, a, np.float32(-(2.0 ** -80))) == np.float32(1 + 2.0 ** -11)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((3, 3000)).astype(np.float32)
+    x[2, :1000] = -(x[0, :1000] * x[1, :1000])                 # cancellation
+    x[2, 1000:2000] = x[2, 1000:2000] * np.float32(2.0 ** -40)  # ties seen only exactly
+    got = fma32(x[0], x[1], x[2])
+    for i in range(x.shape[1]):
+        v = Fraction(float(x[0, i])) * Fraction(float(x[1, i])) + Fraction(float(x[2, i]))
+        assert got[i] == _round_f32_exact(v), i
+
+
 def test_neighbour_counts_equal_brute_force():
     case = cases.perturb(cases.small_column_3d(nx=12, ny=6, nz=14, seed=8), seed=9)
     p, st, g = sorted_oracle(case)
@@ -92,7 +124,7 @@ def test_neighbour_counts_equal_brute_force():
     ih = np.float32(p.i_h)
     for i in range(p.npb, st.n):
         d = f[i] - f
-        r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        r2 = r2_fp32(d)
         hit = (np.sqrt(r2) * ih) < np.float32(2.0)
         hit[i] = False
         assert nbr[i] == hit.sum(), i
