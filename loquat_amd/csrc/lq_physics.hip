@@ -516,10 +516,22 @@ struct PairIn {
     float p;  // pressure of j, read only with artificial pressure
 };
 
+// The value at byte offset off from base.
+template <class T>
+__device__ __forceinline__ T ldb(const T* __restrict__ base, uint32_t off) {
+    return *(const T*)((const char*)base + off);
+}
+
 template <bool ART = false>
 __device__ __forceinline__ PairIn pair_load(const DevState& st, uint32_t j) {
     const float pj = ART ? ldo(reinterpret_cast<const float*>(st.pospres) + 3, 4u * j) : 0.0f;
-    return PairIn{ldo(st.xd, j), ldo(st.yd, j), ldo(st.zd, j), ldo(st.velrhop, j), ldo(st.str, j), pj};
+    // one shift serves every offset: 8 j, 16 j and 24 j = 8 j + 16 j (the
+    // empty asm keeps the compiler from folding 24 j back into a 32-bit
+    // multiply, a quarter-rate instruction)
+    uint32_t o8 = j << 3;
+    asm("" : "+v"(o8));
+    const uint32_t o16 = o8 << 1;
+    return PairIn{ldb(st.xd, o8), ldb(st.yd, o8), ldb(st.zd, o8), ldb(st.velrhop, o16), ldb(st.str, o8 + o16), pj};
 }
 
 // Pair body of variants 3 to 6: as pair_eval, with 32-bit offset loads and
@@ -548,8 +560,9 @@ __device__ __forceinline__ void pair_compute(const IPart& pi, const PairIn& pj, 
     a.L[6] += uz * gx;
     a.L[7] += uz * gy;
     a.L[8] += uz * gz;
-    a.dvg += dvx * gx + dvy * gy + dvz * gz;
     const float vdx = -(dvx * dx + dvy * dy + dvz * dz);
+    // (v_j - v_i).grad W = -fac (v_i - v_j).(x_i - x_j)
+    a.dvg -= fac * vdx;
     const float mu = c.h * fminf(vdx, 0.0f) * frcp(r2 + c.eta);
     float pij = -2.0f * c.alpha_cs * mu * frcp(pi.rho + vr.w);
     const float ir2 = ir * ir;
