@@ -522,16 +522,24 @@ __device__ __forceinline__ T ldb(const T* __restrict__ base, uint32_t off) {
     return *(const T*)((const char*)base + off);
 }
 
-template <bool ART = false>
-__device__ __forceinline__ PairIn pair_load(const DevState& st, uint32_t j) {
-    const float pj = ART ? ldo(reinterpret_cast<const float*>(st.pospres) + 3, 4u * j) : 0.0f;
-    // one shift serves every offset: 8 j, 16 j and 24 j = 8 j + 16 j (the
-    // empty asm keeps the compiler from folding 24 j back into a 32-bit
-    // multiply, a quarter-rate instruction)
+// Byte offsets of particle j in the 8-, 16- and 24-byte arrays from one
+// shift: 24 j = 8 j + 16 j (the empty asm keeps the compiler from folding
+// it back into a 32-bit multiply, a quarter-rate instruction).
+struct Offs {
+    uint32_t o8, o16, o24;
+};
+__device__ __forceinline__ Offs offs_of(uint32_t j) {
     uint32_t o8 = j << 3;
     asm("" : "+v"(o8));
     const uint32_t o16 = o8 << 1;
-    return PairIn{ldb(st.xd, o8), ldb(st.yd, o8), ldb(st.zd, o8), ldb(st.velrhop, o16), ldb(st.str, o8 + o16), pj};
+    return Offs{o8, o16, o8 + o16};
+}
+
+template <bool ART = false>
+__device__ __forceinline__ PairIn pair_load(const DevState& st, uint32_t j) {
+    const float pj = ART ? ldo(reinterpret_cast<const float*>(st.pospres) + 3, 4u * j) : 0.0f;
+    const Offs o = offs_of(j);
+    return PairIn{ldb(st.xd, o.o8), ldb(st.yd, o.o8), ldb(st.zd, o.o8), ldb(st.velrhop, o.o16), ldb(st.str, o.o24), pj};
 }
 
 // Pair body of variants 3 to 6: as pair_eval, with 32-bit offset loads and
@@ -1247,16 +1255,19 @@ __global__ __launch_bounds__(256) void k_adami_lds(DP dp, DevState st, const int
         cnt = scan_compact<1, DP::padded, IC2_CAP, STALE, U, PF, ORD>(
             on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane, on ? (uint32_t)k : 0u,
             [&](uint32_t j, bool act) {
-                add(ldo(st.xd, j), ldo(st.yd, j), ldo(st.zd, j), ldo(st.velrhop, j).w, ldo(st.str, j), act);
+                const Offs o = offs_of(j);
+                add(ldb(st.xd, o.o8), ldb(st.yd, o.o8), ldb(st.zd, o.o8), ldb(st.velrhop, o.o16).w, ldb(st.str, o.o24),
+                    act);
             });
     } else {
         cnt = scan_compact<2, DP::padded, IC2_CAP, STALE, U, PF, ORD>(
             on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane, on ? (uint32_t)k : 0u,
             [&](uint32_t j0, bool a0, uint32_t j1, bool a1) {
-                const double x0 = ldo(st.xd, j0), y0 = ldo(st.yd, j0), z0 = ldo(st.zd, j0);
-                const double x1 = ldo(st.xd, j1), y1 = ldo(st.yd, j1), z1 = ldo(st.zd, j1);
-                const float r0 = ldo(st.velrhop, j0).w, r1 = ldo(st.velrhop, j1).w;
-                const tensor2d s0 = ldo(st.str, j0), s1 = ldo(st.str, j1);
+                const Offs o0 = offs_of(j0), o1 = offs_of(j1);
+                const double x0 = ldb(st.xd, o0.o8), y0 = ldb(st.yd, o0.o8), z0 = ldb(st.zd, o0.o8);
+                const double x1 = ldb(st.xd, o1.o8), y1 = ldb(st.yd, o1.o8), z1 = ldb(st.zd, o1.o8);
+                const float r0 = ldb(st.velrhop, o0.o16).w, r1 = ldb(st.velrhop, o1.o16).w;
+                const tensor2d s0 = ldb(st.str, o0.o24), s1 = ldb(st.str, o1.o24);
                 add(x0, y0, z0, r0, s0, a0);
                 add(x1, y1, z1, r1, s1, a1);
             });
@@ -1761,14 +1772,15 @@ __global__ __launch_bounds__(256) void k_shepard_lds(DP dp, DevState st, const t
     float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f, wsum = 0.f;
     scan_compact<1, DP::padded>(on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane, on ? (uint32_t)k : 0u,
                              [&](uint32_t j, bool act) {
-        const float dx = (float)(xi - ldo(st.xd, j)), dy = (float)(yi - ldo(st.yd, j)),
-                    dz = (float)(zi - ldo(st.zd, j));
+        const Offs o = offs_of(j);
+        const float dx = (float)(xi - ldb(st.xd, o.o8)), dy = (float)(yi - ldb(st.yd, o.o8)),
+                    dz = (float)(zi - ldb(st.zd, o.o8));
         const float q = fsqrt(dx * dx + dy * dy + dz * dz) * i_h;
         const float tt = fmaxf(2.0f - q, 0.0f);
         const float t2 = tt * tt;
-        const float kv0 = cw * t2 * t2 * (q + 0.5f) * m * frcp(ldo(st.velrhop, j).w);
+        const float kv0 = cw * t2 * t2 * (q + 0.5f) * m * frcp(ldb(st.velrhop, o.o16).w);
         const float kv = act ? kv0 : 0.0f;
-        const tensor2d s = ldo(src, j);
+        const tensor2d s = ldb(src, o.o24);
         sxx += s.xx * kv;
         syy += s.yy * kv;
         szz += s.zz * kv;
