@@ -440,7 +440,7 @@ lq_status lq_engine_neighbour_counts_stale(lq_engine* eng, int32_t run_id, int32
  * more than 0.45 cell between builds is stopped as failed.  Only the order of
  * the FP32 pair sums differs between intervals.  1 .. LQ_MAX_SORT_INTERVAL;
  * default LQ_DEFAULT_SORT_INTERVAL. */
-#define LQ_DEFAULT_SORT_INTERVAL 8
+#define LQ_DEFAULT_SORT_INTERVAL 12
 #define LQ_MAX_SORT_INTERVAL 16
 lq_status lq_engine_set_sort_interval(lq_engine* eng, int32_t steps);
 int32_t lq_engine_get_sort_interval(const lq_engine* eng);

@@ -90,7 +90,7 @@ INTERACTION_VARIANTS = 20  # LQ_INTERACTION_VARIANTS
 CELL_ORDER_CANDIDATES = 6  # LQ_CELL_ORDER_CANDIDATES
 DEFAULT_INTERACTION_VARIANT = 19  # LQ_DEFAULT_INTERACTION_VARIANT
 COLUMN_ORDER_CANDIDATES = 5  # LQ_COLUMN_ORDER_CANDIDATES
-DEFAULT_SORT_INTERVAL = 8  # LQ_DEFAULT_SORT_INTERVAL
+DEFAULT_SORT_INTERVAL = 12  # LQ_DEFAULT_SORT_INTERVAL
 MAX_SORT_INTERVAL = 16  # LQ_MAX_SORT_INTERVAL
 MAX_PARTICLES = 178956970  # LQ_MAX_PARTICLES
 NOALLOC_MAX_PARTICLES = 2097152  # LQ_NOALLOC_MAX_PARTICLES

@@ -254,7 +254,7 @@ GOLDEN_C2 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "
 @pytest.mark.skipif(not os.path.exists(GOLDEN_C2), reason="oracle full-run fixture not generated")
 def test_c2_column_full_run_matches_oracle(gpu):
     """The benchmark's 1.55 M-particle 3D column, all 5,000 steps on the GPU
-    (default engine settings: a neighbour search every 8 steps, z-binned
+    (default engine settings: a neighbour search every 12 steps, z-binned
     cells, the default interaction variant): runout, centre of mass, height and free-surface profile within
     1 % of the FP64 oracle's full run (tests/golden/c2_column_trajectory.json,
     written by tests/golden/make_c2_trajectory.py)."""
