@@ -394,7 +394,11 @@ struct RunSlot {
 
 struct Buffers {
     DevState S[2];
-    DevIni I;  // written by the predictor, read by the corrector (and, positions, by the stale scans)
+    DevIni I;  // written by the predictor, read by the corrector
+    // Variant 19 (split integration): the half step's state is I's positions,
+    // velocity and stress with hpos as its FP32 mirror; the step's start stays
+    // in the state and is the corrector's ini set.
+    float4* hpos;
     DevRates R;
     tensor2d* T;
     int32_t* run_of;
@@ -540,12 +544,23 @@ int enqueue_step(lq_engine* e, int cur, bool shepard, int sorts, Prof* prof) {
     int c = cur;
     const IndexMap walls{b.wall_idx, 0}, mats{b.mat_idx, 0};
     const int nw = (int)e->nwall_total, nm = (int)e->nmat_total;
+    // variant 19: the predictor writes the half step beside the step's start
+    // (no ini copy); the corrector half's scans read it
+    const bool split = var == 19;
     for (int half = 0; half < 2; ++half) {
         const bool corr = half == 1;
         if (!corr && sorts > 0) {
             StageTimer t(prof, s, PG_GRID);
             c = enqueue_sort(e, c, sorts == 2 ? REKEY_ALL : REKEY_SORT_DUE, nullptr, s);
         }
+        DevState hs = b.S[c];
+        hs.xd = b.I.xd;
+        hs.yd = b.I.yd;
+        hs.zd = b.I.zd;
+        hs.pospres = b.hpos;
+        hs.velrhop = b.I.velrhop;
+        hs.str = b.I.str;
+        const DevState& sc = (split && corr) ? hs : b.S[c];  // what this half's scans read
         // predictor half: the tables kept from earlier steps (from_desc 1:
         // the runs re-keyed just now have bound 0; 2: none was); corrector
         // half: the table of this step's start and the predictor's bound
@@ -560,18 +575,22 @@ int enqueue_step(lq_engine* e, int cur, bool shepard, int sorts, Prof* prof) {
         const StaleIn* sp = (corr || sorts != 2) ? &stale : nullptr;
         {
             StageTimer t(prof, s, PG_ADAMI);
-            launch_adami_table(b.desc, b.run_of, b.S[c], b.grid.cell_beg, b.grid.cell_end, walls, nw, var, sp, s);
+            launch_adami_table(b.desc, b.run_of, sc, b.grid.cell_beg, b.grid.cell_end, walls, nw, var, sp, s,
+                               (split && corr) ? b.S[c].str : nullptr);
         }
         {
             StageTimer t(prof, s, PG_INTER);
-            launch_interaction_table(b.desc, b.run_of, b.S[c], b.R, b.grid.cell_beg, b.grid.cell_end, mats, nm,
+            launch_interaction_table(b.desc, b.run_of, sc, b.R, b.grid.cell_beg, b.grid.cell_end, mats, nm,
                                      (int)e->nmat_art, nullptr, var, sp, s);
         }
         {
             StageTimer t(prof, s, PG_INTEG);
             launch_integrate_table(corr, b.desc, b.run_of, b.S[c], b.I, b.R, mats, nm, b.fail, b.dmax, b.nout, s,
-                                   b.dmaxb, b.noutb);
-            if (!corr) launch_dmax_fold(b.dmax, nr, s);
+                                   b.dmaxb, b.noutb, split ? &hs : nullptr);
+            if (!corr) {
+                launch_dmax_fold(b.dmax, nr, s);
+                if (split) launch_walls_to_half(walls, nw, b.S[c], hs, s);
+            }
         }
     }
     if (shepard) {
@@ -786,7 +805,8 @@ extern "C" lq_status lq_engine_create(int64_t np_cap, int64_t ncell_cap, int32_t
         DevIni& I = b.I;
         ok = ok && dmalloc(I.xd, np_cap, e->owned) && dmalloc(I.yd, np_cap, e->owned) &&
              dmalloc(I.zd, np_cap, e->owned) && dmalloc(I.velrhop, np_cap, e->owned) &&
-             dmalloc(I.epvr, np_cap, e->owned) && dmalloc(I.str, np_cap, e->owned) && dmalloc(I.stn, np_cap, e->owned);
+             dmalloc(I.epvr, np_cap, e->owned) && dmalloc(I.str, np_cap, e->owned) && dmalloc(I.stn, np_cap, e->owned) &&
+             dmalloc(b.hpos, np_cap + SCAN_PAD, e->owned);
     }
     ok = ok && dmalloc(b.R.accad, np_cap, e->owned) && dmalloc(b.R.spin, np_cap, e->owned) &&
          dmalloc(b.R.acestn, np_cap, e->owned) && dmalloc(b.T, np_cap, e->owned) &&

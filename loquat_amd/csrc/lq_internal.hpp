@@ -101,9 +101,10 @@ enum Stage { ST_ADAMI = 0, ST_INTERACT = 1, ST_PREDICT = 2, ST_CORRECT = 3, ST_S
 // of a fresh table (list-based kernels only; nbr: per-particle counts).
 void launch_adami_one(const RunDesc& d, const DevState& st, const int32_t* beg, const int32_t* end, int variant,
                       const StaleIn* stale, int32_t* nbr, hipStream_t s);
+// str2 (variant 19 only): also writes the walls' stress there.
 void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const int32_t* beg,
                         const int32_t* end, IndexMap walls, int nwalls, int variant, const StaleIn* stale,
-                        hipStream_t s);
+                        hipStream_t s, tensor2d* str2 = nullptr);
 void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates& rt, const int32_t* beg,
                             const int32_t* end, int32_t* nbr, int variant, const StaleIn* stale, hipStream_t s);
 // The last nart of the nmat material indices belong to runs with
@@ -122,10 +123,17 @@ void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, 
 // stale table tolerates.
 // dmaxb / noutb (nullable, corrector): the same for where the step leaves the
 // particles relative to their run's cell table (engine states).
+// half (non-null: split mode): the predictor leaves st as the step's start
+// and writes the half step's positions, velocity and stress to *half; the
+// corrector reads its ini set from st and the half step's pressure from
+// *half (same arithmetic, no ini copy).
 void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
                             const DevIni& ini, const DevRates& rt, IndexMap mats, int nmat, int32_t* fail,
                             uint32_t* dmax, uint32_t* nout, hipStream_t s, uint32_t* dmaxb = nullptr,
-                            uint32_t* noutb = nullptr);
+                            uint32_t* noutb = nullptr, const DevState* half = nullptr);
+// Walls' positions, FP32 mirror, velocity and stress from st into the
+// half-step state.
+void launch_walls_to_half(IndexMap walls, int nwalls, const DevState& st, const DevState& half, hipStream_t s);
 // Shepard filter: reads stress from str_src, writes str_dst (walls and runs
 // not due are copied through).
 void launch_shepard_one(const RunDesc& d, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,

@@ -1196,10 +1196,13 @@ k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ be
 // either (the same sums in the same order): a wave of walls next to the
 // material is bound by its chain of dependent gathers, one round per flush
 // iteration.
+// str2 (nullable): a second copy of the walls' stress (split integration:
+// the step's state beside the half-step state the scans read).
 template <class DP, bool STALE = false, int WPB = 4, int PF = 0, int ORD = 0, int B = 1, int U = IC2_U>
 __global__ __launch_bounds__(256) void k_adami_lds(DP dp, DevState st, const int32_t* __restrict__ beg,
                                                    const int32_t* __restrict__ end, IndexMap walls, int n,
-                                                   StaleIn sn = {}, int32_t* __restrict__ nbr = nullptr) {
+                                                   StaleIn sn = {}, int32_t* __restrict__ nbr = nullptr,
+                                                   tensor2d* __restrict__ str2 = nullptr) {
     __shared__ int32_t lists[WPB][IC2_CAP][64];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;  // round-robin XCDs, as k_adami
     const int k = t < n ? walls(t) : -1;
@@ -1285,6 +1288,7 @@ __global__ __launch_bounds__(256) void k_adami_lds(DP dp, DevState st, const int
         o.xz = sxz * iw;
         o.yz = syz * iw;
         st.str[k] = o;
+        if (str2) str2[k] = o;
     }
 }
 
@@ -1524,8 +1528,12 @@ __device__ __forceinline__ float moved_since_table(float4 po, double x, double y
 // Predictor (src/predictor_corrector.cu:169-218) fused with the *_ini copy
 // that the reference does as five device-to-device copies (:240-246), for
 // one material particle; returns how far it moved (rounded up).
+// SPLIT (engine): the step's start stays in st, which the corrector then
+// reads as its ini set, and the half step's positions, velocity and stress
+// go to h (nothing is copied to ini, idep and stn are left as they were).
+template <bool SPLIT = false>
 __device__ __forceinline__ float predict_one(const RunDesc& d, int k, const DevState& st, const DevIni& ini,
-                                             const DevRates& rt, bool& outside) {
+                                             const DevRates& rt, bool& outside, const DevState& h = DevState{}) {
     const float dt = d.par.dt;
     const float hdt = 0.5f * dt;
     const double hdtd = (double)hdt;
@@ -1535,13 +1543,15 @@ __device__ __forceinline__ float predict_one(const RunDesc& d, int k, const DevS
     const tensor2d sig0 = st.str[k];
     const tensor2d stn0 = st.stn[k];
     double x = st.xd[k], y = st.yd[k], z = st.zd[k];
-    ini.xd[k] = x;
-    ini.yd[k] = y;
-    ini.zd[k] = z;
-    ini.velrhop[k] = vr;
-    ini.epvr[k] = make_float2(idep.y, idep.z);
-    ini.str[k] = sig0;
-    ini.stn[k] = stn0;
+    if constexpr (!SPLIT) {
+        ini.xd[k] = x;
+        ini.yd[k] = y;
+        ini.zd[k] = z;
+        ini.velrhop[k] = vr;
+        ini.epvr[k] = make_float2(idep.y, idep.z);
+        ini.str[k] = sig0;
+        ini.stn[k] = stn0;
+    }
 
     vr.x += hdt * a.x;
     vr.y += hdt * a.y;
@@ -1557,16 +1567,19 @@ __device__ __forceinline__ float predict_one(const RunDesc& d, int k, const DevS
     tensor2d s = sig0;
     constitutive(d, hdt, rt.spin[k], D, stn, p, ep, voidr, s);
 
-    st.xd[k] = x;
-    st.yd[k] = y;
-    st.zd[k] = z;
-    st.pospres[k] = make_float4((float)x, (float)y, (float)z, p);
-    st.velrhop[k] = vr;
-    idep.y = ep;
-    idep.z = voidr;
-    st.idep[k] = idep;
-    st.str[k] = s;
-    st.stn[k] = stn;
+    const DevState& o = SPLIT ? h : st;
+    o.xd[k] = x;
+    o.yd[k] = y;
+    o.zd[k] = z;
+    o.pospres[k] = make_float4((float)x, (float)y, (float)z, p);
+    o.velrhop[k] = vr;
+    if constexpr (!SPLIT) {
+        idep.y = ep;
+        idep.z = voidr;
+        st.idep[k] = idep;
+        st.stn[k] = stn;
+    }
+    o.str[k] = s;
     const float i_h = d.par.i_h;
     const int cx = cell_coord((float)x, i_h), cy = cell_coord((float)y, i_h), cz = cell_coord((float)z, i_h);
     outside = cx < 0 || cx >= d.par.cell_num_x || cy < 0 || cy >= d.par.cell_num_y || cz < 0 ||
@@ -1581,9 +1594,10 @@ __device__ __forceinline__ float predict_one(const RunDesc& d, int k, const DevS
 // particles, for the corrector half's scans over this half step's (now
 // stale) table, folded into slot 0 by k_dmax_fold.  nout (with dmax, per
 // run): set when a particle of the run ends outside the grid.
-template <class DP>
+template <class DP, bool SPLIT>
 __global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni ini, DevRates rt, IndexMap mats,
-                                                   int n, uint32_t* __restrict__ dmax, uint32_t* __restrict__ nout) {
+                                                   int n, uint32_t* __restrict__ dmax, uint32_t* __restrict__ nout,
+                                                   DevState h) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int run = -1;
     float disp = 0.0f;
@@ -1594,7 +1608,7 @@ __global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni in
         if ((k - d.p_off) >= d.npb && d.active) {
             run = r;
             bool outside = false;
-            disp = predict_one(d, k, st, ini, rt, outside);
+            disp = predict_one<SPLIT>(d, k, st, ini, rt, outside, h);
             if (outside && nout) atomicOr(&nout[r], 1u);
         }
     }
@@ -1618,27 +1632,33 @@ __global__ void k_dmax_fold(uint32_t* __restrict__ dmax, int nruns) {
 // not finite, so a diverging run can be stopped without touching the others.
 // Returns how far the particle is from where its cell table put it (engine
 // states; 0 otherwise) and whether it ended outside the grid.
+// SPLIT (engine): the predictor left the step's start in st and wrote the
+// half step to h, so the ini values are read from st and the half step's
+// pressure from h.
+template <bool SPLIT = false>
 __device__ __forceinline__ float correct_one(const RunDesc& d, int run, int k, const DevState& st, const DevIni& ini,
                                              const DevRates& rt, int32_t* __restrict__ fail,
-                                             const uint32_t* __restrict__ dmax, bool& outside) {
+                                             const uint32_t* __restrict__ dmax, bool& outside,
+                                             const DevState& h = DevState{}) {
     const float dt = d.par.dt;
     const double dtd = (double)dt;
     const float4 a = rt.accad[k];
-    const float4 v0 = ini.velrhop[k];
+    float4 idep = st.idep[k];
+    const float4 v0 = SPLIT ? st.velrhop[k] : ini.velrhop[k];
     float4 vr;
     vr.x = v0.x + dt * a.x;
     vr.y = v0.y + dt * a.y;
     vr.z = v0.z + dt * a.z;
     vr.w = v0.w + dt * a.w;
-    const double x = 0.5 * ((double)vr.x + (double)v0.x) * dtd + ini.xd[k];
-    const double y = 0.5 * ((double)vr.y + (double)v0.y) * dtd + ini.yd[k];
-    const double z = 0.5 * ((double)vr.z + (double)v0.z) * dtd + ini.zd[k];
+    const double x = 0.5 * ((double)vr.x + (double)v0.x) * dtd + (SPLIT ? st.xd[k] : ini.xd[k]);
+    const double y = 0.5 * ((double)vr.y + (double)v0.y) * dtd + (SPLIT ? st.yd[k] : ini.yd[k]);
+    const double z = 0.5 * ((double)vr.z + (double)v0.z) * dtd + (SPLIT ? st.zd[k] : ini.zd[k]);
     const tensor2d D = rt.acestn[k];
-    const tensor2d stn = tadd(ini.stn[k], tscale(D, dt));
-    float p = st.pospres[k].w;
-    const float2 epvr = ini.epvr[k];
+    const tensor2d stn = tadd(SPLIT ? st.stn[k] : ini.stn[k], tscale(D, dt));
+    float p = SPLIT ? h.pospres[k].w : st.pospres[k].w;
+    const float2 epvr = SPLIT ? make_float2(idep.y, idep.z) : ini.epvr[k];
     float ep = epvr.x, voidr = epvr.y;
-    tensor2d s = ini.str[k];
+    tensor2d s = SPLIT ? st.str[k] : ini.str[k];
     constitutive(d, dt, rt.spin[k], D, stn, p, ep, voidr, s);
     if (fail) {
         const float chk = (vr.x + vr.y + vr.z + vr.w) + (s.xx + s.yy + s.zz) + (s.xy + s.xz + s.yz);
@@ -1654,7 +1674,6 @@ __device__ __forceinline__ float correct_one(const RunDesc& d, int run, int k, c
     st.zd[k] = z;
     st.pospres[k] = make_float4((float)x, (float)y, (float)z, p);
     st.velrhop[k] = vr;
-    float4 idep = st.idep[k];
     idep.y = ep;
     idep.z = voidr;
     st.idep[k] = idep;
@@ -1670,12 +1689,11 @@ __device__ __forceinline__ float correct_one(const RunDesc& d, int run, int k, c
 // dmaxb / noutb (nullable, as dmax / nout of k_predictor): where the step
 // leaves each run's particles relative to its cell table, for the next
 // step's scans when the table is kept.
-template <class DP>
+template <class DP, bool SPLIT>
 __global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni ini, DevRates rt, IndexMap mats,
                                                    int n, int32_t* __restrict__ fail,
-                                                   const uint32_t* __restrict__ dmax = nullptr,
-                                                   uint32_t* __restrict__ dmaxb = nullptr,
-                                                   uint32_t* __restrict__ noutb = nullptr) {
+                                                   const uint32_t* __restrict__ dmax, uint32_t* __restrict__ dmaxb,
+                                                   uint32_t* __restrict__ noutb, DevState h) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int run = -1;
     float disp = 0.0f;
@@ -1686,7 +1704,7 @@ __global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni in
         if ((k - d.p_off) >= d.npb && d.active) {
             run = r;
             bool outside = false;
-            disp = correct_one(d, r, k, st, ini, rt, fail, dmax, outside);
+            disp = correct_one<SPLIT>(d, r, k, st, ini, rt, fail, dmax, outside, h);
             if (outside && noutb) atomicOr(&noutb[r], 1u);
         }
     }
@@ -1923,15 +1941,17 @@ __global__ void k_copy_tensor(const tensor2d* __restrict__ src, tensor2d* __rest
 // Variants 3 and up use the list-based neighbour scan, the others the direct one.
 template <class DP>
 static void launch_adami(const DP& dp, const DevState& st, const int32_t* beg, const int32_t* end, IndexMap walls,
-                         int n, int variant, const StaleIn* stale, int32_t* nbr, hipStream_t s) {
+                         int n, int variant, const StaleIn* stale, int32_t* nbr, hipStream_t s,
+                         tensor2d* str2 = nullptr) {
     if (n <= 0) return;
     if (variant == 19) {
         // variant 18's shape gathering two pairs at a time (same sums)
         if (stale)
-            k_adami_lds<DP, true, 1, 16, 2, 2, 8><<<blocks_for(n, 64), 64, 0, s>>>(dp, st, beg, end, walls, n, *stale, nbr);
+            k_adami_lds<DP, true, 1, 16, 2, 2, 8>
+                <<<blocks_for(n, 64), 64, 0, s>>>(dp, st, beg, end, walls, n, *stale, nbr, str2);
         else
             k_adami_lds<DP, false, 1, 16, 2, 2, 8>
-                <<<blocks_for(n, 64), 64, 0, s>>>(dp, st, beg, end, walls, n, StaleIn{}, nbr);
+                <<<blocks_for(n, 64), 64, 0, s>>>(dp, st, beg, end, walls, n, StaleIn{}, nbr, str2);
         return;
     }
     if (variant >= 12) {
@@ -1956,8 +1976,8 @@ void launch_adami_one(const RunDesc& d, const DevState& st, const int32_t* beg, 
 }
 void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const int32_t* beg,
                         const int32_t* end, IndexMap walls, int nwalls, int variant, const StaleIn* stale,
-                        hipStream_t s) {
-    launch_adami(DescTable{d, run_of}, st, beg, end, walls, nwalls, variant, stale, nullptr, s);
+                        hipStream_t s, tensor2d* str2) {
+    launch_adami(DescTable{d, run_of}, st, beg, end, walls, nwalls, variant, stale, nullptr, s, str2);
 }
 
 // Variant 11's shape (64-thread workgroups, 40-entry lists, four candidates
@@ -2079,20 +2099,53 @@ void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, 
     if (n <= 0) return;
     const IndexMap mats{nullptr, d.npb};
     if (corrector)
-        k_corrector<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, mats, n, nullptr);
+        k_corrector<DescOne, false><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, mats, n, nullptr,
+                                                                       nullptr, nullptr, nullptr, DevState{});
     else
-        k_predictor<DescOne><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, mats, n, nullptr, nullptr);
+        k_predictor<DescOne, false><<<blocks_for(n, 256), 256, 0, s>>>(DescOne{d}, st, ini, rt, mats, n, nullptr,
+                                                                       nullptr, DevState{});
 }
 void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run_of, const DevState& st,
                             const DevIni& ini, const DevRates& rt, IndexMap mats, int nmat, int32_t* fail,
-                            uint32_t* dmax, uint32_t* nout, hipStream_t s, uint32_t* dmaxb, uint32_t* noutb) {
+                            uint32_t* dmax, uint32_t* nout, hipStream_t s, uint32_t* dmaxb, uint32_t* noutb,
+                            const DevState* half) {
     if (nmat <= 0) return;
     const int nb = blocks_for(nmat, 256);
-    if (corrector)
-        k_corrector<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat, fail, dmax, dmaxb,
-                                                  noutb);
-    else
-        k_predictor<DescTable><<<nb, 256, 0, s>>>(DescTable{d, run_of}, st, ini, rt, mats, nmat, dmax, nout);
+    const DescTable dp{d, run_of};
+    if (corrector) {
+        if (half)
+            k_corrector<DescTable, true><<<nb, 256, 0, s>>>(dp, st, ini, rt, mats, nmat, fail, dmax, dmaxb, noutb,
+                                                            *half);
+        else
+            k_corrector<DescTable, false><<<nb, 256, 0, s>>>(dp, st, ini, rt, mats, nmat, fail, dmax, dmaxb, noutb,
+                                                             DevState{});
+    } else {
+        if (half)
+            k_predictor<DescTable, true><<<nb, 256, 0, s>>>(dp, st, ini, rt, mats, nmat, dmax, nout, *half);
+        else
+            k_predictor<DescTable, false><<<nb, 256, 0, s>>>(dp, st, ini, rt, mats, nmat, dmax, nout, DevState{});
+    }
+}
+
+namespace {
+// Walls' positions, velocities and stresses into the half-step state (walls
+// do not move; the corrector half's Adami pass then rewrites the stress of
+// the walls next to material, and the others' is what a wall's own unused
+// list slots gather, as in the state itself).
+__global__ __launch_bounds__(256) void k_walls_to_half(IndexMap walls, int n, DevState st, DevState h) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int k = walls(t);
+    h.xd[k] = st.xd[k];
+    h.yd[k] = st.yd[k];
+    h.zd[k] = st.zd[k];
+    h.pospres[k] = st.pospres[k];
+    h.velrhop[k] = st.velrhop[k];
+    h.str[k] = st.str[k];
+}
+}  // namespace
+void launch_walls_to_half(IndexMap walls, int nwalls, const DevState& st, const DevState& half, hipStream_t s) {
+    if (nwalls > 0) k_walls_to_half<<<blocks_for(nwalls, 256), 256, 0, s>>>(walls, nwalls, st, half);
 }
 void launch_dmax_fold(uint32_t* dmax, int nruns, hipStream_t s) {
     if (nruns > 0) k_dmax_fold<<<blocks_for(nruns, 64), 64, 0, s>>>(dmax, nruns);
