@@ -397,7 +397,8 @@ struct Buffers {
     DevIni I;  // written by the predictor, read by the corrector
     // Variant 19 (split integration): the half step's state is I's positions,
     // velocity and stress with hpos as its FP32 mirror; the step's start stays
-    // in the state and is the corrector's ini set.
+    // in the state and is the corrector's ini set.  Its wall entries are
+    // written by each build's permute.
     float4* hpos;
     DevRates R;
     tensor2d* T;
@@ -493,8 +494,16 @@ int enqueue_sort(lq_engine* e, int cur, int which, tensor2d* str_dst, hipStream_
     Buffers& b = e->b;
     const int n = (int)e->np_total;
     launch_grid_build_table(b.desc, b.run_of, b.S[cur].pospres, n, e->ncell_total, zbits_eff(e), b.grid, s, which);
+    // the walls also go to the half-step state of split integration
+    DevState half{};
+    half.xd = b.I.xd;
+    half.yd = b.I.yd;
+    half.zd = b.I.zd;
+    half.pospres = b.hpos;
+    half.velrhop = b.I.velrhop;
+    half.str = b.I.str;
     launch_permute(b.grid.vals[b.grid.sorted_slot], n, b.S[cur], b.S[1 - cur], nullptr, nullptr, str_dst, s, b.desc,
-                   b.run_of, which);
+                   b.run_of, which, &half);
     return 1 - cur;
 }
 
@@ -587,10 +596,7 @@ int enqueue_step(lq_engine* e, int cur, bool shepard, int sorts, Prof* prof) {
             StageTimer t(prof, s, PG_INTEG);
             launch_integrate_table(corr, b.desc, b.run_of, b.S[c], b.I, b.R, mats, nm, b.fail, b.dmax, b.nout, s,
                                    b.dmaxb, b.noutb, split ? &hs : nullptr);
-            if (!corr) {
-                launch_dmax_fold(b.dmax, nr, s);
-                if (split) launch_walls_to_half(walls, nw, b.S[c], hs, s);
-            }
+            if (!corr) launch_dmax_fold(b.dmax, nr, s);
         }
     }
     if (shepard) {

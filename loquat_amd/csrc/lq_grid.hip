@@ -298,9 +298,13 @@ __global__ void k_cell_table(DP dp, const uint32_t* __restrict__ keys, int n, in
 
 // With table positions (engine states): particles of re-keyed runs record
 // the position their keys came from; the others keep theirs.
+// half (engine, nullable arrays): the walls' positions, FP32 mirror,
+// velocity and stress are also written there, in the new order (walls do not
+// move between builds, so the half-step state of split integration keeps
+// them from one build to the next).
 __global__ __launch_bounds__(256) void k_permute(const uint32_t* __restrict__ perm, int n, DevState src, DevState dst,
                                                  int with_ini, DevIni isrc, DevIni idst, tensor2d* str_dst,
-                                                 DescTable dp, int which) {
+                                                 DescTable dp, int which, DevState half) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int s = (int)perm[k];
@@ -314,6 +318,17 @@ __global__ __launch_bounds__(256) void k_permute(const uint32_t* __restrict__ pe
     (str_dst ? str_dst : dst.str)[k] = src.str[s];
     dst.stn[k] = src.stn[s];
     dst.orig[k] = src.orig[s];
+    if (half.xd) {
+        const RunDesc& d = dp.get(dp.run(k));
+        if (k - d.p_off < d.npb) {
+            half.xd[k] = src.xd[s];
+            half.yd[k] = src.yd[s];
+            half.zd[k] = src.zd[s];
+            half.pospres[k] = src.pospres[s];
+            half.velrhop[k] = src.velrhop[s];
+            half.str[k] = src.str[s];
+        }
+    }
     if (with_ini) {
         idst.xd[k] = isrc.xd[s];
         idst.yd[k] = isrc.yd[s];
@@ -496,13 +511,14 @@ lq_status launch_grid_build_noalloc(const RunDesc& d, const float4* pospres, int
 
 void launch_permute(const uint32_t* perm, int n, const DevState& src, const DevState& dst, const DevIni* ini_src,
                     const DevIni* ini_dst, tensor2d* str_dst, hipStream_t s, const RunDesc* desc,
-                    const int32_t* run_of, int which) {
+                    const int32_t* run_of, int which, const DevState* half) {
     if (n <= 0) return;
     const int with_ini = (ini_src && ini_dst) ? 1 : 0;
     DevIni a = with_ini ? *ini_src : DevIni{};
     DevIni b = with_ini ? *ini_dst : DevIni{};
     k_permute<<<blocks_for(n, 256), 256, 0, s>>>(perm, n, src, dst, with_ini, a, b, str_dst,
-                                                 DescTable{desc, run_of}, which);
+                                                 DescTable{desc, run_of}, which,
+                                                 (half && desc) ? *half : DevState{});
 }
 
 }  // namespace lq

@@ -83,9 +83,11 @@ lq_status launch_grid_build_noalloc(const RunDesc& d, const float4* pospres, int
 // str_dst is non-null the permuted stress goes there instead of dst.str.
 // Engine states also carry table positions: those of the runs `which`
 // re-keyed (desc / run_of) become the positions just sorted.
+// half (engine, with desc): also writes the walls' positions, FP32 mirror,
+// velocity and stress into that half-step state.
 void launch_permute(const uint32_t* perm, int n, const DevState& src, const DevState& dst, const DevIni* ini_src,
                     const DevIni* ini_dst, tensor2d* str_dst, hipStream_t s, const RunDesc* desc = nullptr,
-                    const int32_t* run_of = nullptr, int which = REKEY_ALL);
+                    const int32_t* run_of = nullptr, int which = REKEY_ALL, const DevState* half = nullptr);
 
 // ---------------------------------------------------------------- physics
 enum Stage { ST_ADAMI = 0, ST_INTERACT = 1, ST_PREDICT = 2, ST_CORRECT = 3, ST_SHEPARD = 4 };
@@ -131,9 +133,7 @@ void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run
                             const DevIni& ini, const DevRates& rt, IndexMap mats, int nmat, int32_t* fail,
                             uint32_t* dmax, uint32_t* nout, hipStream_t s, uint32_t* dmaxb = nullptr,
                             uint32_t* noutb = nullptr, const DevState* half = nullptr);
-// Walls' positions, FP32 mirror, velocity and stress from st into the
-// half-step state.
-void launch_walls_to_half(IndexMap walls, int nwalls, const DevState& st, const DevState& half, hipStream_t s);
+
 // Shepard filter: reads stress from str_src, writes str_dst (walls and runs
 // not due are copied through).
 void launch_shepard_one(const RunDesc& d, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,

@@ -2127,26 +2127,6 @@ void launch_integrate_table(bool corrector, const RunDesc* d, const int32_t* run
     }
 }
 
-namespace {
-// Walls' positions, velocities and stresses into the half-step state (walls
-// do not move; the corrector half's Adami pass then rewrites the stress of
-// the walls next to material, and the others' is what a wall's own unused
-// list slots gather, as in the state itself).
-__global__ __launch_bounds__(256) void k_walls_to_half(IndexMap walls, int n, DevState st, DevState h) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int k = walls(t);
-    h.xd[k] = st.xd[k];
-    h.yd[k] = st.yd[k];
-    h.zd[k] = st.zd[k];
-    h.pospres[k] = st.pospres[k];
-    h.velrhop[k] = st.velrhop[k];
-    h.str[k] = st.str[k];
-}
-}  // namespace
-void launch_walls_to_half(IndexMap walls, int nwalls, const DevState& st, const DevState& half, hipStream_t s) {
-    if (nwalls > 0) k_walls_to_half<<<blocks_for(nwalls, 256), 256, 0, s>>>(walls, nwalls, st, half);
-}
 void launch_dmax_fold(uint32_t* dmax, int nruns, hipStream_t s) {
     if (nruns > 0) k_dmax_fold<<<blocks_for(nruns, 64), 64, 0, s>>>(dmax, nruns);
 }
