@@ -1760,12 +1760,16 @@ __global__ __launch_bounds__(256) void k_shepard(DP dp, DevState st, const tenso
 }
 
 // Shepard filter with the list-based scan; same sums as k_shepard, same order.
-template <class DP>
+// WPB waves per workgroup; PF (partial flushes), B (gathers per flush round)
+// and U (candidates per round) as in scan_compact, none of which changes a
+// sum; XCD: XCD-contiguous blocks (otherwise round-robin, which balances the
+// walls, who only copy).
+template <class DP, int WPB = 4, int PF = 0, int B = 1, int U = IC2_U, bool XCD = false>
 __global__ __launch_bounds__(256) void k_shepard_lds(DP dp, DevState st, const tensor2d* __restrict__ src,
                                                      tensor2d* __restrict__ dst, const int32_t* __restrict__ beg,
                                                      const int32_t* __restrict__ end, int n) {
-    __shared__ int32_t lists[4][IC2_CAP][64];
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;  // walls copy only: round-robin XCDs balance the work
+    __shared__ int32_t lists[WPB][IC2_CAP][64];
+    const int k = XCD ? xcd_thread() : (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     bool on = false;
     float i_h = 0.f, cw = 0.f, m = 0.f, r2cut = 0.f;
@@ -1788,17 +1792,14 @@ __global__ __launch_bounds__(256) void k_shepard_lds(DP dp, DevState st, const t
         }
     }
     float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f, wsum = 0.f;
-    scan_compact<1, DP::padded>(on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane, on ? (uint32_t)k : 0u,
-                             [&](uint32_t j, bool act) {
-        const Offs o = offs_of(j);
-        const float dx = (float)(xi - ldb(st.xd, o.o8)), dy = (float)(yi - ldb(st.yd, o.o8)),
-                    dz = (float)(zi - ldb(st.zd, o.o8));
+    // one pair's sums from its gathered data
+    auto add = [&](double xj, double yj, double zj, float rhoj, const tensor2d& s, bool act) {
+        const float dx = (float)(xi - xj), dy = (float)(yi - yj), dz = (float)(zi - zj);
         const float q = fsqrt(dx * dx + dy * dy + dz * dz) * i_h;
         const float tt = fmaxf(2.0f - q, 0.0f);
         const float t2 = tt * tt;
-        const float kv0 = cw * t2 * t2 * (q + 0.5f) * m * frcp(ldb(st.velrhop, o.o16).w);
+        const float kv0 = cw * t2 * t2 * (q + 0.5f) * m * frcp(rhoj);
         const float kv = act ? kv0 : 0.0f;
-        const tensor2d s = ldb(src, o.o24);
         sxx += s.xx * kv;
         syy += s.yy * kv;
         szz += s.zz * kv;
@@ -1806,7 +1807,28 @@ __global__ __launch_bounds__(256) void k_shepard_lds(DP dp, DevState st, const t
         sxz += s.xz * kv;
         syz += s.yz * kv;
         wsum += kv;
-    });
+    };
+    if constexpr (B == 1) {
+        scan_compact<1, DP::padded, IC2_CAP, false, U, PF>(
+            on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane, on ? (uint32_t)k : 0u,
+            [&](uint32_t j, bool act) {
+                const Offs o = offs_of(j);
+                add(ldb(st.xd, o.o8), ldb(st.yd, o.o8), ldb(st.zd, o.o8), ldb(st.velrhop, o.o16).w, ldb(src, o.o24),
+                    act);
+            });
+    } else {
+        scan_compact<2, DP::padded, IC2_CAP, false, U, PF>(
+            on, 1, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane, on ? (uint32_t)k : 0u,
+            [&](uint32_t j0, bool a0, uint32_t j1, bool a1) {
+                const Offs o0 = offs_of(j0), o1 = offs_of(j1);
+                const double x0 = ldb(st.xd, o0.o8), y0 = ldb(st.yd, o0.o8), z0 = ldb(st.zd, o0.o8);
+                const double x1 = ldb(st.xd, o1.o8), y1 = ldb(st.yd, o1.o8), z1 = ldb(st.zd, o1.o8);
+                const float r0 = ldb(st.velrhop, o0.o16).w, r1 = ldb(st.velrhop, o1.o16).w;
+                const tensor2d s0 = ldb(src, o0.o24), s1 = ldb(src, o1.o24);
+                add(x0, y0, z0, r0, s0, a0);
+                add(x1, y1, z1, r1, s1, a1);
+            });
+    }
     if (k >= n) return;
     if (on && wsum > 0.0f) {
         const float iw = 1.0f / wsum;
@@ -2135,7 +2157,13 @@ template <class DP>
 static void launch_shepard(const DP& dp, const DevState& st, const tensor2d* str_src, tensor2d* str_dst,
                            const int32_t* beg, const int32_t* end, int n, int variant, hipStream_t s) {
     if (n <= 0) return;
-    if (variant >= 3)
+    if (variant == 19)
+        // one-wave workgroups, partial flushes, two gathers per round and
+        // eight candidates per round (the sums of variant 18's shape); blocks
+        // round-robin over the XCDs (XCD-contiguous blocks left the XCDs
+        // holding walls idle: 0.041 against 0.029 ms per C2 step)
+        k_shepard_lds<DP, 1, 16, 2, 8, false><<<blocks_for(n, 64), 64, 0, s>>>(dp, st, str_src, str_dst, beg, end, n);
+    else if (variant >= 3)
         k_shepard_lds<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, str_src, str_dst, beg, end, n);
     else
         k_shepard<DP><<<blocks_for(n, 256), 256, 0, s>>>(dp, st, str_src, str_dst, beg, end, n);

@@ -180,16 +180,19 @@ def test_interaction_variants_agree_in_engine(gpu):
 def test_staged_scans_match_unstaged_at_full_size(gpu):
     """Variant 19 stages the interaction's candidate chunks through LDS from
     524,288 material particles per launch (the 1.55 M-particle column has
-    1,024,000); variant 18 is the same scan unstaged.  On a perturbed column,
-    over ten steps (kept tables and one re-sort), every result bit agrees."""
+    1,024,000), integrates split (the half step beside the step's start, no
+    ini copy) and runs Adami and Shepard in other shapes; variant 18 is the
+    same scan unstaged with the reference's ini copy.  On a perturbed column,
+    over 22 steps (kept tables, a re-sort and a Shepard filter at step 20),
+    every result bit agrees."""
     from loquat_amd import abi
-    c = cases.perturb(cases.column_3d(nsteps=10), seed=61)
-    assert c.np - c.derived().npb >= 1 << 19
+    c = cases.perturb(cases.column_3d(nsteps=22), seed=61)
+    assert c.np - c.derived().npb >= 1 << 19 and c.par.stress_shepard_filter == 20
     res = []
     for v in (18, 19):
         abi.check(abi.lib().lq_set_interaction_variant(v))
-        e, (rid,) = _engine([c], 10)
-        e.step(10)
+        e, (rid,) = _engine([c], 22)
+        e.step(22)
         res.append(e.download(rid))
         e.close()
     abi.check(abi.lib().lq_set_interaction_variant(abi.DEFAULT_INTERACTION_VARIANT))
