@@ -35,6 +35,17 @@ FP32_VALU_PEAK_TFLOPS = 78.6   # non-packed FP32 vector (packed FP32 is disabled
 FLOPS_PER_PAIR = 80            # algorithmic FP32 flops per neighbour pair
 BYTES_READ_PER_PARTICLE = 64   # x (FP64, 24) + v (12) + rho (4) + stress (24)
 BYTES_WRITE_PER_MATERIAL = 52  # a (12) + drho (4) + spin (12) + strain rate (24)
+# With the integration fused into the interaction (variant 19 on runs without
+# artificial pressure or a dynamic time step) no rates are written; instead,
+# per material particle and half step, beyond what the scan reads:
+# predictor: reads ep, void ratio (8), strain (24), p (4), table position
+# (12); writes x (FP64, 24), FP32 position + p (16), v + rho (16), stress (24)
+# = 128 B; corrector: reads the step's start x (24), v + rho (16), stress
+# (24), strain (24), ep, void ratio (8), p (4), table position (12); writes
+# x (24), position + p (16), v + rho (16), ep, void ratio (8), stress (24),
+# strain (24) = 224 B.  Mean per launch: 176 B.
+BYTES_INTEG_PER_MATERIAL = 176
+FUSED_VARIANTS = (19,)
 BYTES_PER_CELL = 8             # cell_beg + cell_end
 WORKLOAD_ID = "column_3d dx=5mm npm=1024000"  # the key of this workload in profiles/*.json
 # Template arguments of k_interaction_v3 per variant (3 and up): pairs per
@@ -51,11 +62,13 @@ V3_SHAPES = {3: (1, False, 40, 4, 4, 0, 0, 1, False), 4: (2, False, 40, 4, 4, 0,
              19: (2, False, 32, 4, 1, 16, 2, 1, True)}
 
 
-def v3_name(shape: tuple, stale: bool) -> str:
+def v3_name(shape: tuple, stale: bool, fuse: int = 0) -> str:
+    """Demangled name of one k_interaction_v3 instance (fuse 1: with the
+    predictor fused in, 2: the corrector)."""
     b, masked, cap, u, wpb, pf, order, lpp, stg = shape
     f = lambda x: "true" if x else "false"  # noqa: E731
     return (f"k_interaction_v3<DescTable, {b}, {f(masked)}, false, {f(stale)}, {cap}, {u}, {wpb}, {pf}, {order}, "
-            f"{lpp}, {f(stg)}>")
+            f"{lpp}, {f(stg)}, {fuse}>")
 
 
 KERNEL_NAMES = {0: "k_interaction_direct", 1: "k_interaction_lds", 2: "k_interaction_unrolled",
@@ -73,7 +86,18 @@ def stale_kernel(variant: int) -> str:
 
 
 def roofline_kernel(variant: int) -> str:
+    if variant in FUSED_VARIANTS:
+        sh = V3_SHAPES[variant]
+        return (f"{v3_name(sh, True, 1)} + {v3_name(sh, True, 2)} (kept tables, predictor and corrector "
+                f"fused in) + {v3_name(sh, False, 1)} (fresh table at sorts)")
     return f"{stale_kernel(variant)} (kept tables) + {KERNEL_NAMES[variant]} (fresh table at sorts)"
+
+
+def bytes_per_launch(variant: int, np_: int, npm: int, ncell: int) -> int:
+    """Algorithmic HBM bytes of one interaction launch (with the fused
+    integration where the variant has it)."""
+    per_mat = BYTES_INTEG_PER_MATERIAL if variant in FUSED_VARIANTS else BYTES_WRITE_PER_MATERIAL
+    return np_ * BYTES_READ_PER_PARTICLE + npm * per_mat + ncell * BYTES_PER_CELL
 
 
 def log(msg: str) -> None:
@@ -585,7 +609,7 @@ def main() -> None:
     inter_launch_ms = stage[2] / 2.0   # two interaction launches per step
     log("stage ms/step: grid %.3f adami %.3f interaction %.3f integrate %.3f shepard %.3f" % tuple(stage))
 
-    bytes_launch = case.np * BYTES_READ_PER_PARTICLE + case.npm * BYTES_WRITE_PER_MATERIAL + p.ncell * BYTES_PER_CELL
+    bytes_launch = bytes_per_launch(variant, case.np, case.npm, p.ncell)
     flops_launch = pairs * FLOPS_PER_PAIR
     gbs = bytes_launch / (inter_launch_ms * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(roofline_kernel(variant))
@@ -629,12 +653,17 @@ def main() -> None:
             "kernel": roofline_kernel(variant),
             "measured_by": "HIP events around the interaction launches on the engine stream (lq_engine_profile); "
                            "launch_ms is the mean of the step's two launches",
-            "bound": "valu",
-            "achieved": round(tflops, 3),
-            "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
+            # the bound whose time at the published peak is the longer
+            "bound": "hbm" if t_hbm >= t_valu else "valu",
+            "achieved": round(gbs, 2) if t_hbm >= t_valu else round(tflops, 3),
+            "peak": HBM_PEAK_GBS if t_hbm >= t_valu else FP32_PEAK_TFLOPS,
+            "unit": "GB/s" if t_hbm >= t_valu else "TFLOP/s",
             "frac": round(max(t_hbm, t_valu) / launch_s, 5),
-            "frac_definition": "max(bytes / 8 TB/s, flops / 157.3 TFLOP/s) / launch time (FP32 flops bind)",
+            "frac_definition": "max(bytes / 8 TB/s, flops / 157.3 TFLOP/s) / launch time (with the integration "
+                               "fused in, its bytes are counted, its flops are not)",
+            "valu": {"achieved_tflops": round(tflops, 3), "peak_tflops": FP32_PEAK_TFLOPS,
+                     "frac": round(t_valu / launch_s, 5)},
+            "fused_integration": variant in FUSED_VARIANTS,
             "frac_nonpacked": round(max(t_hbm, t_valu_np) / launch_s, 5),
             "peak_nonpacked": FP32_VALU_PEAK_TFLOPS,
             "traffic": traffic,

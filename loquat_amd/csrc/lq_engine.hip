@@ -554,8 +554,11 @@ int enqueue_step(lq_engine* e, int cur, bool shepard, int sorts, Prof* prof) {
     const IndexMap walls{b.wall_idx, 0}, mats{b.mat_idx, 0};
     const int nw = (int)e->nwall_total, nm = (int)e->nmat_total;
     // variant 19: the predictor writes the half step beside the step's start
-    // (no ini copy); the corrector half's scans read it
+    // (no ini copy); the corrector half's scans read it.  Without artificial
+    // pressure or a dynamic time step (which reads the rates), the
+    // interaction kernel integrates its particles itself.
     const bool split = var == 19;
+    const bool fuse = split && e->nmat_art == 0 && !e->any_dynamic;
     for (int half = 0; half < 2; ++half) {
         const bool corr = half == 1;
         if (!corr && sorts > 0) {
@@ -587,15 +590,28 @@ int enqueue_step(lq_engine* e, int cur, bool shepard, int sorts, Prof* prof) {
             launch_adami_table(b.desc, b.run_of, sc, b.grid.cell_beg, b.grid.cell_end, walls, nw, var, sp, s,
                                (split && corr) ? b.S[c].str : nullptr);
         }
+        FuseIn fz{};
+        if (corr) {
+            fz.other = b.S[c];
+            fz.fail = b.fail;
+            fz.dmax_pred = b.dmax;
+            fz.dmax = b.dmaxb;
+            fz.nout = b.noutb;
+        } else {
+            fz.other = hs;
+            fz.dmax = b.dmax;
+            fz.nout = b.nout;
+        }
         {
             StageTimer t(prof, s, PG_INTER);
             launch_interaction_table(b.desc, b.run_of, sc, b.R, b.grid.cell_beg, b.grid.cell_end, mats, nm,
-                                     (int)e->nmat_art, nullptr, var, sp, s);
+                                     (int)e->nmat_art, nullptr, var, sp, s, fuse ? &fz : nullptr, corr);
         }
         {
             StageTimer t(prof, s, PG_INTEG);
-            launch_integrate_table(corr, b.desc, b.run_of, b.S[c], b.I, b.R, mats, nm, b.fail, b.dmax, b.nout, s,
-                                   b.dmaxb, b.noutb, split ? &hs : nullptr);
+            if (!fuse)
+                launch_integrate_table(corr, b.desc, b.run_of, b.S[c], b.I, b.R, mats, nm, b.fail, b.dmax, b.nout,
+                                       s, b.dmaxb, b.noutb, split ? &hs : nullptr);
             if (!corr) launch_dmax_fold(b.dmax, nr, s);
         }
     }

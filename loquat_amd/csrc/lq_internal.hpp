@@ -112,9 +112,22 @@ void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates
 // The last nart of the nmat material indices belong to runs with
 // artificial-pressure = 1; they get a kernel of their own so that the other
 // runs keep the exact arithmetic of the plain kernel.
+// fuse (variant 19 in split mode, nart = 0): the predictor's (corr: the
+// corrector's) half step is done by the interaction kernel itself, as
+// launch_integrate_table would do it, and rt is left untouched.
+struct FuseIn {
+    // the scans read st: the step's start in the predictor half, the half
+    // step in the corrector half; other is the other one
+    DevState other;
+    int32_t* fail;              // corrector: per-run failure flags
+    const uint32_t* dmax_pred;  // corrector: the predictor's folded bound
+    uint32_t* dmax;             // DMAX_SLOTS displacement slots per run
+    uint32_t* nout;             // per run: a particle ended outside the grid
+};
 void launch_interaction_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt,
                               const int32_t* beg, const int32_t* end, IndexMap mats, int nmat, int nart,
-                              int32_t* nbr, int variant, const StaleIn* stale, hipStream_t s);
+                              int32_t* nbr, int variant, const StaleIn* stale, hipStream_t s,
+                              const FuseIn* fuse = nullptr, bool corr = false);
 void launch_integrate_one(bool corrector, const RunDesc& d, const DevState& st, const DevIni& ini,
                           const DevRates& rt, hipStream_t s);
 // fail: per-run flags raised by the corrector on a non-finite state

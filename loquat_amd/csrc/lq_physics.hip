@@ -234,21 +234,39 @@ __device__ __forceinline__ IConst make_iconst(const RunDesc& d) {
     return c;
 }
 
-__device__ __forceinline__ void store_rates(const RunDesc& d, const DevRates& rt, int k, const IAcc& a) {
+// One particle's rates: acceleration and density rate, spin, strain rate.
+struct RateVals {
+    float4 accad;
+    float4 spin;
+    tensor2d acestn;
+};
+
+__device__ __forceinline__ RateVals rates_of(const RunDesc& d, const IAcc& a) {
     const float m = d.par.m;
-    rt.accad[k] = make_float4(m * a.ax + d.par.acc_x, m * a.ay + d.par.acc_y, m * a.az + d.par.acc_z, -m * a.dvg);
+    RateVals r;
+    r.accad = make_float4(m * a.ax + d.par.acc_x, m * a.ay + d.par.acc_y, m * a.az + d.par.acc_z, -m * a.dvg);
     const float hm = 0.5f * m;
     // spin = (W_yx, W_zx, W_zy): sign fixed by the Jaumann term of
     // src/predictor_corrector.cu:40-45 (rigid rotation about z gives spin.x = Omega).
-    rt.spin[k] = make_float4(hm * (a.L[3] - a.L[1]), hm * (a.L[6] - a.L[2]), hm * (a.L[7] - a.L[5]), 0.0f);
-    tensor2d e;
-    e.xx = m * a.L[0];
-    e.yy = m * a.L[4];
-    e.zz = m * a.L[8];
-    e.xy = hm * (a.L[1] + a.L[3]);
-    e.xz = hm * (a.L[2] + a.L[6]);
-    e.yz = hm * (a.L[5] + a.L[7]);
-    rt.acestn[k] = e;
+    r.spin = make_float4(hm * (a.L[3] - a.L[1]), hm * (a.L[6] - a.L[2]), hm * (a.L[7] - a.L[5]), 0.0f);
+    r.acestn.xx = m * a.L[0];
+    r.acestn.yy = m * a.L[4];
+    r.acestn.zz = m * a.L[8];
+    r.acestn.xy = hm * (a.L[1] + a.L[3]);
+    r.acestn.xz = hm * (a.L[2] + a.L[6]);
+    r.acestn.yz = hm * (a.L[5] + a.L[7]);
+    return r;
+}
+
+__device__ __forceinline__ void store_rates(const RunDesc& d, const DevRates& rt, int k, const IAcc& a) {
+    const RateVals r = rates_of(d, a);
+    rt.accad[k] = r.accad;
+    rt.spin[k] = r.spin;
+    rt.acestn[k] = r.acestn;
+}
+
+__device__ __forceinline__ RateVals load_rates(const DevRates& rt, int k) {
+    return RateVals{rt.accad[k], rt.spin[k], rt.acestn[k]};
 }
 
 // All interaction variants run n threads over the material particles of all
@@ -1087,108 +1105,6 @@ __device__ __forceinline__ int scan_compact(bool active, int layer_lo, int layer
     return count;
 }
 
-// Variant 3 (B = 1): variant 2's structure with fewer instructions per
-// candidate and per pair, pruned scans, and XCD-contiguous blocks.  The
-// particle itself stays in its candidate stream (it is always a hit when it
-// lies inside the grid): all its pair terms are exactly zero (x_i - x_i = 0
-// zeroes the kernel gradient and v_i - v_i the velocity terms), so it changes
-// no sum, and one is taken off the neighbour count.  Variant 4 (B = 2)
-// evaluates the lists two pairs at a time; same pairs, same order.  ART adds
-// the artificial pressure; STALE scans a table built before the particles
-// last moved (sn).  Kernels without artificial pressure are held to four
-// waves per SIMD (at most 128 VGPRs).
-// Shape parameters searched by the autotuner: CAP list entries per lane,
-// UU candidates per load round, WPB waves per workgroup (lists are per wave,
-// so a workgroup holds WPB x CAP x 64 entries of LDS).
-template <class DP, int B, bool MASKED = false, bool ART = false, bool STALE = false, int CAP = IC3_CAP,
-          int UU = IC2_U, int WPB = 4, int PF = 0, int ORD = 0, int LPP = 1, bool STG = false>
-__global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu((STALE || !ART) ? 4 : 1))) void
-k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ beg, const int32_t* __restrict__ end,
-                 IndexMap mats, int n, int32_t* __restrict__ nbr, StaleIn sn = {}) {
-    __shared__ int32_t lists[WPB][CAP][64];
-    // STG: each wave's chunk of staged candidates
-    __shared__ float4 staged[STG ? WPB : 1][STG ? STG_SLOTS : 1];
-    // LPP lanes per particle: adjacent lanes of one wave
-    const int t = xcd_thread() / LPP;
-    const int k = t < n ? mats(t) : -1;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    bool mat = false;
-    const RunDesc* dptr = nullptr;
-    if (t < n) {
-        const RunDesc& d = dp.get(dp.run(k));
-        dptr = &d;
-        mat = (k - d.p_off) >= d.npb && d.active;
-    }
-    IConst c = {};
-    float r2cut = 0.f;
-    float4 ps = make_float4(0.f, 0.f, 0.f, 0.f);
-    IPart pi = {};
-    ScanGeom g = {};
-    if (mat) {
-        const RunDesc& d = *dptr;
-        c = make_iconst(d);
-        r2cut = d.r2_cut;
-        ps = st.pospres[k];
-        load_part(st, k, pi);
-        if constexpr (STALE) {
-            g = make_geom_stale(d, ps, stale_pos(sn, k), stale_disp(sn, d, dp.run(k)), stale_out_now(sn, d, dp.run(k)));
-        } else {
-            g = make_geom(d, ps);
-        }
-    }
-    // the particle is its own candidate exactly when its current cell lies
-    // inside the grid
-    const int ccx = cell_coord(ps.x, c.i_h), ccy = cell_coord(ps.y, c.i_h), ccz = cell_coord(ps.z, c.i_h);
-    const bool self_in = mat && ccx >= 0 && ccx < g.nx && ccy >= 0 && ccy < g.ny && ccz >= 0 && ccz < g.nz;
-    IAcc a = {};
-    // dummy = the particle itself: its pair terms are exactly zero
-    int count;
-    // MASKED: a lane past the end of its list loads nothing and computes the
-    // self pair from registers (positions and velocity of i, zero stress),
-    // instead of gathering particle i's data again
-    const PairIn self = PairIn{pi.x, pi.y, pi.z, make_float4(pi.vx, pi.vy, pi.vz, pi.rho), tensor2d{}, ps.w};
-    if constexpr (ART) pi.R = c.eps * fmaxf(-ps.w, 0.0f) / (pi.rho * pi.rho);
-    auto load = [&](uint32_t j, bool act) {
-        if constexpr (MASKED) {
-            PairIn q = self;
-            if (act) q = pair_load<ART>(st, j);
-            return q;
-        } else {
-            return pair_load<ART>(st, j);
-        }
-    };
-    if constexpr (B == 1) {
-        count = scan_compact<1, DP::padded, CAP, STALE, UU, PF, ORD, LPP, STG>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
-                                            mat ? (uint32_t)k : 0u,
-                                            [&](uint32_t j, bool act) { pair_compute<ART>(pi, load(j, act), c, a); },
-                                            staged[STG ? w : 0]);
-    } else {
-        count = scan_compact<2, DP::padded, CAP, STALE, UU, PF, ORD, LPP, STG>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
-                                            mat ? (uint32_t)k : 0u,
-                                            [&](uint32_t j0, bool a0, uint32_t j1, bool a1) {
-                                                const PairIn q0 = load(j0, a0);
-                                                const PairIn q1 = load(j1, a1);
-                                                pair_compute<ART>(pi, q0, c, a);
-                                                pair_compute<ART>(pi, q1, c, a);
-                                            }, staged[STG ? w : 0]);
-    }
-    if constexpr (LPP == 2) {
-        // the particle's two partial sums: each lane adds its neighbour's
-        // (lane ^ 1), so both hold the same total
-        a.ax += swap_pair(a.ax);
-        a.ay += swap_pair(a.ay);
-        a.az += swap_pair(a.az);
-        a.dvg += swap_pair(a.dvg);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) a.L[q] += swap_pair(a.L[q]);
-        count += __float_as_int(swap_pair(__int_as_float(count)));
-    }
-    if (mat && (LPP == 1 || (lane & 1) == 0)) {
-        store_rates(*dptr, rt, k, a);
-        if (nbr) nbr[k] = count - (self_in ? 1 : 0);
-    }
-}
-
 // Adami wall stress with the list-based scan (material layer only; a wall is
 // never in its own candidate stream).  Same sums as k_adami, same order.
 // WPB waves per workgroup; PF and ORD as in scan_compact (partial flushes,
@@ -1533,11 +1449,11 @@ __device__ __forceinline__ float moved_since_table(float4 po, double x, double y
 // go to h (nothing is copied to ini, idep and stn are left as they were).
 template <bool SPLIT = false>
 __device__ __forceinline__ float predict_one(const RunDesc& d, int k, const DevState& st, const DevIni& ini,
-                                             const DevRates& rt, bool& outside, const DevState& h = DevState{}) {
+                                             const RateVals& rv, bool& outside, const DevState& h = DevState{}) {
     const float dt = d.par.dt;
     const float hdt = 0.5f * dt;
     const double hdtd = (double)hdt;
-    const float4 a = rt.accad[k];
+    const float4 a = rv.accad;
     float4 vr = st.velrhop[k];
     float4 idep = st.idep[k];
     const tensor2d sig0 = st.str[k];
@@ -1560,12 +1476,12 @@ __device__ __forceinline__ float predict_one(const RunDesc& d, int k, const DevS
     x += hdtd * (double)vr.x;
     y += hdtd * (double)vr.y;
     z += hdtd * (double)vr.z;
-    const tensor2d D = rt.acestn[k];
+    const tensor2d D = rv.acestn;
     const tensor2d stn = tadd(stn0, tscale(D, hdt));
     float p = st.pospres[k].w;
     float ep = idep.y, voidr = idep.z;
     tensor2d s = sig0;
-    constitutive(d, hdt, rt.spin[k], D, stn, p, ep, voidr, s);
+    constitutive(d, hdt, rv.spin, D, stn, p, ep, voidr, s);
 
     const DevState& o = SPLIT ? h : st;
     o.xd[k] = x;
@@ -1608,7 +1524,7 @@ __global__ __launch_bounds__(256) void k_predictor(DP dp, DevState st, DevIni in
         if ((k - d.p_off) >= d.npb && d.active) {
             run = r;
             bool outside = false;
-            disp = predict_one<SPLIT>(d, k, st, ini, rt, outside, h);
+            disp = predict_one<SPLIT>(d, k, st, ini, load_rates(rt, k), outside, h);
             if (outside && nout) atomicOr(&nout[r], 1u);
         }
     }
@@ -1637,12 +1553,12 @@ __global__ void k_dmax_fold(uint32_t* __restrict__ dmax, int nruns) {
 // pressure from h.
 template <bool SPLIT = false>
 __device__ __forceinline__ float correct_one(const RunDesc& d, int run, int k, const DevState& st, const DevIni& ini,
-                                             const DevRates& rt, int32_t* __restrict__ fail,
+                                             const RateVals& rv, int32_t* __restrict__ fail,
                                              const uint32_t* __restrict__ dmax, bool& outside,
                                              const DevState& h = DevState{}) {
     const float dt = d.par.dt;
     const double dtd = (double)dt;
-    const float4 a = rt.accad[k];
+    const float4 a = rv.accad;
     float4 idep = st.idep[k];
     const float4 v0 = SPLIT ? st.velrhop[k] : ini.velrhop[k];
     float4 vr;
@@ -1653,13 +1569,13 @@ __device__ __forceinline__ float correct_one(const RunDesc& d, int run, int k, c
     const double x = 0.5 * ((double)vr.x + (double)v0.x) * dtd + (SPLIT ? st.xd[k] : ini.xd[k]);
     const double y = 0.5 * ((double)vr.y + (double)v0.y) * dtd + (SPLIT ? st.yd[k] : ini.yd[k]);
     const double z = 0.5 * ((double)vr.z + (double)v0.z) * dtd + (SPLIT ? st.zd[k] : ini.zd[k]);
-    const tensor2d D = rt.acestn[k];
+    const tensor2d D = rv.acestn;
     const tensor2d stn = tadd(SPLIT ? st.stn[k] : ini.stn[k], tscale(D, dt));
     float p = SPLIT ? h.pospres[k].w : st.pospres[k].w;
     const float2 epvr = SPLIT ? make_float2(idep.y, idep.z) : ini.epvr[k];
     float ep = epvr.x, voidr = epvr.y;
     tensor2d s = SPLIT ? st.str[k] : ini.str[k];
-    constitutive(d, dt, rt.spin[k], D, stn, p, ep, voidr, s);
+    constitutive(d, dt, rv.spin, D, stn, p, ep, voidr, s);
     if (fail) {
         const float chk = (vr.x + vr.y + vr.z + vr.w) + (s.xx + s.yy + s.zz) + (s.xy + s.xz + s.yz);
         if (!isfinite(chk) || !isfinite(x + y + z)) fail[run] = 1;
@@ -1704,11 +1620,164 @@ __global__ __launch_bounds__(256) void k_corrector(DP dp, DevState st, DevIni in
         if ((k - d.p_off) >= d.npb && d.active) {
             run = r;
             bool outside = false;
-            disp = correct_one<SPLIT>(d, r, k, st, ini, rt, fail, dmax, outside, h);
+            disp = correct_one<SPLIT>(d, r, k, st, ini, load_rates(rt, k), fail, dmax, outside, h);
             if (outside && noutb) atomicOr(&noutb[r], 1u);
         }
     }
     if (dmaxb) block_max_by_run(run, disp, dmaxb, 1, 0);
+}
+
+// Largest value per run over one wave (workgroups of one wave, no shared
+// memory), into slot blockIdx % DMAX_SLOTS of red: when the contributing
+// lanes share a run the wave's maximum is added once, otherwise each lane
+// adds its own.  v >= 0; lanes with run < 0 contribute nothing.  Every lane
+// of the wave must call it.
+__device__ __forceinline__ void wave_max_by_run(int run, float v, uint32_t* red) {
+    const int slot = (int)(blockIdx.x % DMAX_SLOTS);
+    const unsigned long long has = __ballot(run >= 0);
+    if (has == 0ull) return;
+    const int r0 = __shfl(run, __ffsll((long long)has) - 1, 64);
+    if (__ballot(run >= 0 && run != r0) == 0ull) {
+        if (run < 0) v = 0.0f;
+        for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+        if ((threadIdx.x & 63) == 0) atomicMax(&red[(size_t)r0 * DMAX_SLOTS + slot], __float_as_uint(v));
+    } else if (run >= 0) {
+        atomicMax(&red[(size_t)run * DMAX_SLOTS + slot], __float_as_uint(v));
+    }
+}
+
+// The integration half step fused into the interaction's epilogue (FUSE 1:
+// predictor, 2: corrector; FuseIn): the particle's rates go from registers
+// into predict_one / correct_one instead of through the rates arrays.
+// Neither integrator writes anything the scan reads (the predictor writes
+// only the half step, the corrector only the step's start, each at its own
+// particle), so the fused kernel computes what the two kernels compute.
+template <int FUSE>
+struct FuseArg : FuseIn {};
+template <>
+struct FuseArg<0> {};
+
+// Variant 3 (B = 1): variant 2's structure with fewer instructions per
+// candidate and per pair, pruned scans, and XCD-contiguous blocks.  The
+// particle itself stays in its candidate stream (it is always a hit when it
+// lies inside the grid): all its pair terms are exactly zero (x_i - x_i = 0
+// zeroes the kernel gradient and v_i - v_i the velocity terms), so it changes
+// no sum, and one is taken off the neighbour count.  Variant 4 (B = 2)
+// evaluates the lists two pairs at a time; same pairs, same order.  ART adds
+// the artificial pressure; STALE scans a table built before the particles
+// last moved (sn).  Kernels without artificial pressure are held to four
+// waves per SIMD (at most 128 VGPRs).
+// Shape parameters searched by the autotuner: CAP list entries per lane,
+// UU candidates per load round, WPB waves per workgroup (lists are per wave,
+// so a workgroup holds WPB x CAP x 64 entries of LDS).  FUSE (one-wave
+// workgroups, one lane per particle): the integration fused in (FuseIn),
+// nothing written to rt.
+template <class DP, int B, bool MASKED = false, bool ART = false, bool STALE = false, int CAP = IC3_CAP,
+          int UU = IC2_U, int WPB = 4, int PF = 0, int ORD = 0, int LPP = 1, bool STG = false, int FUSE = 0>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu((STALE || !ART) ? 4 : 1))) void
+k_interaction_v3(DP dp, DevState st, DevRates rt, const int32_t* __restrict__ beg, const int32_t* __restrict__ end,
+                 IndexMap mats, int n, int32_t* __restrict__ nbr, StaleIn sn = {}, FuseArg<FUSE> fz = {}) {
+    static_assert(FUSE == 0 || (WPB == 1 && LPP == 1), "fused integration needs one-wave workgroups");
+    __shared__ int32_t lists[WPB][CAP][64];
+    // STG: each wave's chunk of staged candidates
+    __shared__ float4 staged[STG ? WPB : 1][STG ? STG_SLOTS : 1];
+    // LPP lanes per particle: adjacent lanes of one wave
+    const int t = xcd_thread() / LPP;
+    const int k = t < n ? mats(t) : -1;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    bool mat = false;
+    const RunDesc* dptr = nullptr;
+    if (t < n) {
+        const RunDesc& d = dp.get(dp.run(k));
+        dptr = &d;
+        mat = (k - d.p_off) >= d.npb && d.active;
+    }
+    IConst c = {};
+    float r2cut = 0.f;
+    float4 ps = make_float4(0.f, 0.f, 0.f, 0.f);
+    IPart pi = {};
+    ScanGeom g = {};
+    if (mat) {
+        const RunDesc& d = *dptr;
+        c = make_iconst(d);
+        r2cut = d.r2_cut;
+        ps = st.pospres[k];
+        load_part(st, k, pi);
+        if constexpr (STALE) {
+            g = make_geom_stale(d, ps, stale_pos(sn, k), stale_disp(sn, d, dp.run(k)), stale_out_now(sn, d, dp.run(k)));
+        } else {
+            g = make_geom(d, ps);
+        }
+    }
+    // the particle is its own candidate exactly when its current cell lies
+    // inside the grid
+    const int ccx = cell_coord(ps.x, c.i_h), ccy = cell_coord(ps.y, c.i_h), ccz = cell_coord(ps.z, c.i_h);
+    const bool self_in = mat && ccx >= 0 && ccx < g.nx && ccy >= 0 && ccy < g.ny && ccz >= 0 && ccz < g.nz;
+    IAcc a = {};
+    // dummy = the particle itself: its pair terms are exactly zero
+    int count;
+    // MASKED: a lane past the end of its list loads nothing and computes the
+    // self pair from registers (positions and velocity of i, zero stress),
+    // instead of gathering particle i's data again
+    const PairIn self = PairIn{pi.x, pi.y, pi.z, make_float4(pi.vx, pi.vy, pi.vz, pi.rho), tensor2d{}, ps.w};
+    if constexpr (ART) pi.R = c.eps * fmaxf(-ps.w, 0.0f) / (pi.rho * pi.rho);
+    auto load = [&](uint32_t j, bool act) {
+        if constexpr (MASKED) {
+            PairIn q = self;
+            if (act) q = pair_load<ART>(st, j);
+            return q;
+        } else {
+            return pair_load<ART>(st, j);
+        }
+    };
+    if constexpr (B == 1) {
+        count = scan_compact<1, DP::padded, CAP, STALE, UU, PF, ORD, LPP, STG>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+                                            mat ? (uint32_t)k : 0u,
+                                            [&](uint32_t j, bool act) { pair_compute<ART>(pi, load(j, act), c, a); },
+                                            staged[STG ? w : 0]);
+    } else {
+        count = scan_compact<2, DP::padded, CAP, STALE, UU, PF, ORD, LPP, STG>(mat, 0, 1, g, ps, r2cut, beg, end, st.pospres, lists[w], lane,
+                                            mat ? (uint32_t)k : 0u,
+                                            [&](uint32_t j0, bool a0, uint32_t j1, bool a1) {
+                                                const PairIn q0 = load(j0, a0);
+                                                const PairIn q1 = load(j1, a1);
+                                                pair_compute<ART>(pi, q0, c, a);
+                                                pair_compute<ART>(pi, q1, c, a);
+                                            }, staged[STG ? w : 0]);
+    }
+    if constexpr (LPP == 2) {
+        // the particle's two partial sums: each lane adds its neighbour's
+        // (lane ^ 1), so both hold the same total
+        a.ax += swap_pair(a.ax);
+        a.ay += swap_pair(a.ay);
+        a.az += swap_pair(a.az);
+        a.dvg += swap_pair(a.dvg);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) a.L[q] += swap_pair(a.L[q]);
+        count += __float_as_int(swap_pair(__int_as_float(count)));
+    }
+    if constexpr (FUSE == 0) {
+        if (mat && (LPP == 1 || (lane & 1) == 0)) {
+            store_rates(*dptr, rt, k, a);
+            if (nbr) nbr[k] = count - (self_in ? 1 : 0);
+        }
+    } else {
+        int run = -1;
+        float disp = 0.0f;
+        if (mat) {
+            const RunDesc& d = *dptr;
+            run = dp.run(k);
+            bool outside = false;
+            const RateVals rv = rates_of(d, a);
+            if constexpr (FUSE == 1)
+                disp = predict_one<true>(d, k, st, DevIni{}, rv, outside, fz.other);
+            else
+                disp = correct_one<true>(d, run, k, fz.other, DevIni{}, rv, fz.fail, fz.dmax_pred, outside, st);
+            if (outside && fz.nout) atomicOr(&fz.nout[run], 1u);
+            if (nbr) nbr[k] = count - (self_in ? 1 : 0);
+        }
+        if (fz.dmax) wave_max_by_run(run, disp, fz.dmax);
+    }
 }
 
 // ----------------------------------------------------------------- Shepard
@@ -2004,12 +2073,33 @@ void launch_adami_table(const RunDesc* d, const int32_t* run_of, const DevState&
 
 // Variant 11's shape (64-thread workgroups, 40-entry lists, four candidates
 // per round) with partial flushes of PF entries and column order ORD.
-template <int PF, int ORD, int LPP = 1, int P = IC3_CAP, bool STG = false, bool M = true, class DP>
+// F: the fused-integration kernels are built (fz non-null selects them, the
+// predictor's or, with corr, the corrector's; runs without artificial
+// pressure only).
+template <int PF, int ORD, int LPP = 1, int P = IC3_CAP, bool STG = false, bool M = true, bool F = false, class DP>
 static void launch_v11_shape(const DP& dp, const DevState& st, const DevRates& rt, const int32_t* beg,
                              const int32_t* end, IndexMap mats, int n, int32_t* nbr, bool art, const StaleIn* stale,
-                             hipStream_t s) {
+                             hipStream_t s, const FuseIn* fz = nullptr, bool corr = false) {
     constexpr int U = IC2_U;
     const int nb = blocks_for((int64_t)LPP * n, 64);
+    if constexpr (F) {
+        if (fz) {
+            const StaleIn sn = stale ? *stale : StaleIn{};
+            if (stale && corr)
+                k_interaction_v3<DP, 2, M, false, true, P, U, 1, PF, ORD, LPP, STG, 2>
+                    <<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, sn, FuseArg<2>{*fz});
+            else if (corr)
+                k_interaction_v3<DP, 2, M, false, false, P, U, 1, PF, ORD, LPP, STG, 2>
+                    <<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, sn, FuseArg<2>{*fz});
+            else if (stale)
+                k_interaction_v3<DP, 2, M, false, true, P, U, 1, PF, ORD, LPP, STG, 1>
+                    <<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, sn, FuseArg<1>{*fz});
+            else
+                k_interaction_v3<DP, 2, M, false, false, P, U, 1, PF, ORD, LPP, STG, 1>
+                    <<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, sn, FuseArg<1>{*fz});
+            return;
+        }
+    }
     if (stale && art)
         k_interaction_v3<DP, 2, M, true, true, P, U, 1, PF, ORD, LPP, STG><<<nb, 64, 0, s>>>(dp, st, rt, beg, end, mats, n, nbr, *stale);
     else if (stale)
@@ -2023,7 +2113,7 @@ static void launch_v11_shape(const DP& dp, const DevState& st, const DevRates& r
 template <class DP>
 static void launch_interaction(const DP& dp, const DevState& st, const DevRates& rt, const int32_t* beg,
                                const int32_t* end, IndexMap mats, int n, int32_t* nbr, int variant, bool art,
-                               const StaleIn* stale, hipStream_t s) {
+                               const StaleIn* stale, hipStream_t s, const FuseIn* fz = nullptr, bool corr = false) {
     if (n <= 0) return;
     const int nb = blocks_for(n, 256);
     // variants 12 to 19: variant 11 with partial flushes of PF entries and
@@ -2045,8 +2135,10 @@ static void launch_interaction(const DP& dp, const DevState& st, const DevRates&
             // launches large enough to fill the GPU many times over (a small
             // launch's time is one wave's latency, which staging lengthens)
             if (n >= STG_MIN_LAUNCH)
-                return launch_v11_shape<16, 2, 1, 32, true, false>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
-            return launch_v11_shape<16, 2, 1, IC3_CAP, false, false>(dp, st, rt, beg, end, mats, n, nbr, art, stale, s);
+                return launch_v11_shape<16, 2, 1, 32, true, false, true>(dp, st, rt, beg, end, mats, n, nbr, art,
+                                                                         stale, s, fz, corr);
+            return launch_v11_shape<16, 2, 1, IC3_CAP, false, false, true>(dp, st, rt, beg, end, mats, n, nbr, art,
+                                                                           stale, s, fz, corr);
         default: break;
     }
     // stale tables: variant 6's shape, in the workgroup size of variants 10
@@ -2107,9 +2199,10 @@ void launch_interaction_one(const RunDesc& d, const DevState& st, const DevRates
 }
 void launch_interaction_table(const RunDesc* d, const int32_t* run_of, const DevState& st, const DevRates& rt,
                               const int32_t* beg, const int32_t* end, IndexMap mats, int nmat, int nart,
-                              int32_t* nbr, int variant, const StaleIn* stale, hipStream_t s) {
+                              int32_t* nbr, int variant, const StaleIn* stale, hipStream_t s, const FuseIn* fuse,
+                              bool corr) {
     const DescTable dt{d, run_of};
-    launch_interaction(dt, st, rt, beg, end, mats, nmat - nart, nbr, variant, false, stale, s);
+    launch_interaction(dt, st, rt, beg, end, mats, nmat - nart, nbr, variant, false, stale, s, fuse, corr);
     if (nart > 0)
         launch_interaction(dt, st, rt, beg, end, IndexMap{mats.idx + (nmat - nart), 0}, nart, nbr, variant, true,
                            stale, s);
