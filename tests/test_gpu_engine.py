@@ -79,7 +79,9 @@ def test_engine_vs_oracle_steps(gpu, model):
 def test_engine_artificial_pressure_vs_oracle(gpu):
     """20 steps with artificial-pressure = 1, beside a run without it in
     the same batch: a cohesive Drucker-Prager column whose top layers start
-    in isotropic tension."""
+    in isotropic tension.  The batch integrates in separate kernels (runs
+    with artificial pressure keep them), the run without the term alone
+    inside its interaction launches: the two give the same bits."""
     case = gentle_case(1, seed=31)
     case.par.model_par_5 = 4.0e3          # DP-kcohesion: tension is admissible
     top = case.posd[:, 2] > np.quantile(case.posd[case.par.npb:, 2], 0.8)
